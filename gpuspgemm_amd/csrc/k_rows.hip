@@ -1,0 +1,368 @@
+// Row-parallel passes: counter reset, NIP + symbolic binning, scatter into
+// bin lists, and the two-kernel exclusive scan that turns nnz per row into
+// row pointers while binning rows for the fused pass.
+//
+// One workgroup handles one tile of kTileRows rows of one product; with a
+// single product tiles are implicit, with a batch they come from a table.
+#include "spg_kernels.hpp"
+
+namespace spg {
+
+__global__ __launch_bounds__(64) void k_clear(DevMeta* meta, int from_word) {
+    int* w = reinterpret_cast<int*>(meta);
+    constexpr int kWords = sizeof(DevMeta) / sizeof(int);
+    for (int i = from_word + threadIdx.x; i < kWords; i += 64) w[i] = 0;
+}
+
+// NIP_i = sum over k in A(i,:) of nnz(B(k,:)) (setup.cuh:48-91): a thread
+// per row, or the whole wave on one row when a row of the wave has more
+// than 32 entries.  Rows with NIP 0 get nnz 0 right away and join no bin
+// (binning.cuh:60-66).
+template <typename V>
+__global__ __launch_bounds__(kRowThreads) void k_nip(ProbSet<V> ps, BinTable sym, DevMeta* meta) {
+    __shared__ int s_cnt[kMaxBins];
+    __shared__ int s_max;
+    __shared__ unsigned long long s_total;
+    __shared__ int s_flags;
+    if (threadIdx.x < kMaxBins) s_cnt[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        s_max = 0;
+        s_total = 0;
+        s_flags = 0;
+    }
+    __syncthreads();
+
+    const Tile t = ps.tile(blockIdx.x);
+    const Prob<V> P = ps.get(t.inst);
+    int my_max = 0;
+    long long my_total = 0;
+    int my_flags = 0;
+#pragma unroll
+    for (int j = 0; j < kTileRows / kRowThreads; ++j) {
+        const int r = threadIdx.x + j * kRowThreads;
+        const bool valid = r < t.rows;
+        const int row = t.row0 + r;
+        int a0 = 0, a1 = 0;
+        if (valid) {
+            a0 = P.a_rpt[row];
+            a1 = P.a_rpt[row + 1];
+        }
+        long long s = 0;
+        if (a1 - a0 <= 32) {
+            for (int p = a0; p < a1; ++p) {
+                const int k = P.a_col[p];
+                s += P.b_rpt[k + 1] - P.b_rpt[k];
+            }
+        }
+        unsigned long long longs = __ballot(a1 - a0 > 32);
+        while (longs) {
+            const int src = __ffsll(static_cast<long long>(longs)) - 1;
+            longs &= longs - 1;
+            const int la0 = __shfl(a0, src);
+            const int la1 = __shfl(a1, src);
+            long long part = 0;
+            for (int p = la0 + lane_id(); p < la1; p += kWave) {
+                const int k = P.a_col[p];
+                part += P.b_rpt[k + 1] - P.b_rpt[k];
+            }
+            part = wave_sum64(part);
+            if (lane_id() == src) s = part;
+        }
+        if (valid) {
+            if (s > INT_MAX) {
+                my_flags |= kFlagNipOverflow;
+                s = INT_MAX;
+            }
+            const int nip = static_cast<int>(s);
+            P.nip[row] = nip;
+            if (nip == 0)
+                P.c_rpt[row] = 0;
+            else
+                atomicAdd(&s_cnt[find_bin(sym, nip, 0)], 1);
+            my_max = max(my_max, nip);
+            my_total += s;
+        }
+    }
+    my_max = wave_max(my_max);
+    my_total = wave_sum64(my_total);
+    if (lane_id() == 0) {
+        atomicMax(&s_max, my_max);
+        atomicAdd(&s_total, static_cast<unsigned long long>(my_total));
+    }
+    if (my_flags) atomicOr(&s_flags, my_flags);
+    __syncthreads();
+    if (threadIdx.x < sym.n && s_cnt[threadIdx.x] > 0)
+        atomicAdd(&meta->sym_count[threadIdx.x], s_cnt[threadIdx.x]);
+    if (threadIdx.x == 0) {
+        atomicMax(&meta->max_nip, s_max);
+        atomicAdd(&meta->total_nip, s_total);
+        if (s_flags) atomicOr(&meta->flags, s_flags);
+    }
+}
+
+// Scatter rows into the symbolic bin lists (binning.cuh:76-120), with the
+// bin offsets derived on the device from the final counts, so no host
+// round trip is needed.
+template <typename V>
+__global__ __launch_bounds__(kRowThreads) void k_scatter(ProbSet<V> ps, BinTable sym, DevMeta* meta,
+                                                         int2* __restrict__ items) {
+    __shared__ int s_cnt[kMaxBins];
+    __shared__ int s_base[kMaxBins];
+    if (threadIdx.x < kMaxBins) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const Tile t = ps.tile(blockIdx.x);
+    const Prob<V> P = ps.get(t.inst);
+    int bin[kTileRows / kRowThreads];
+    int rank[kTileRows / kRowThreads];
+#pragma unroll
+    for (int j = 0; j < kTileRows / kRowThreads; ++j) {
+        const int r = threadIdx.x + j * kRowThreads;
+        bin[j] = -1;
+        if (r < t.rows) {
+            const int nip = P.nip[t.row0 + r];
+            if (nip > 0) {
+                bin[j] = find_bin(sym, nip, 0);
+                rank[j] = atomicAdd(&s_cnt[bin[j]], 1);
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < sym.n && s_cnt[threadIdx.x] > 0) {
+        int off = 0;
+        for (int b = 0; b < static_cast<int>(threadIdx.x); ++b) off += meta->sym_count[b];
+        s_base[threadIdx.x] = off + atomicAdd(&meta->sym_cursor[threadIdx.x], s_cnt[threadIdx.x]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kTileRows / kRowThreads; ++j)
+        if (bin[j] >= 0)
+            items[s_base[bin[j]] + rank[j]] = make_int2(t.inst, t.row0 + threadIdx.x + j * kRowThreads);
+}
+
+// Scan part 1: per-tile sum of nnz, NUM-bin histogram, max nnz and the
+// global-table slots the heaviest rows will need.
+template <typename V>
+__global__ __launch_bounds__(kRowThreads) void k_scan_count(ProbSet<V> ps, BinTable num, DevMeta* meta,
+                                                            long long* __restrict__ tile_sums, int after_fail) {
+    // Rows that overflowed the largest LDS table have no count yet: leave
+    // c_rpt untouched; the host recounts them and runs the scan again.
+    if (!after_fail && meta->fail_count > 0) return;
+    __shared__ int s_cnt[kMaxBins];
+    __shared__ long long s_sum;
+    __shared__ unsigned long long s_pool;
+    __shared__ int s_max;
+    if (threadIdx.x < kMaxBins) s_cnt[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        s_sum = 0;
+        s_pool = 0;
+        s_max = 0;
+    }
+    __syncthreads();
+    const Tile t = ps.tile(blockIdx.x);
+    const Prob<V> P = ps.get(t.inst);
+    long long sum = 0;
+    unsigned long long pool = 0;
+    int mx = 0;
+#pragma unroll
+    for (int j = 0; j < kTileRows / kRowThreads; ++j) {
+        const int r = threadIdx.x + j * kRowThreads;
+        if (r < t.rows) {
+            const int row = t.row0 + r;
+            const int nnz = P.c_rpt[row];
+            sum += nnz;
+            mx = max(mx, nnz);
+            if (nnz > 0) {
+                const int b = find_bin(num, P.nip[row], nnz);
+                atomicAdd(&s_cnt[b], 1);
+                if (num.variant[b] == kNumGlobalVariant) pool += global_table_slots(nnz);
+            }
+        }
+    }
+    sum = wave_sum64(sum);
+    mx = wave_max(mx);
+    long long p64 = wave_sum64(static_cast<long long>(pool));
+    if (lane_id() == 0) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&s_sum), static_cast<unsigned long long>(sum));
+        atomicAdd(&s_pool, static_cast<unsigned long long>(p64));
+        atomicMax(&s_max, mx);
+    }
+    __syncthreads();
+    if (threadIdx.x < num.n && s_cnt[threadIdx.x] > 0)
+        atomicAdd(&meta->num_count[threadIdx.x], s_cnt[threadIdx.x]);
+    if (threadIdx.x == 0) {
+        tile_sums[blockIdx.x] = s_sum;
+        atomicMax(&meta->max_nnz, s_max);
+        if (s_pool) atomicAdd(&meta->gnum_pool, s_pool);
+    }
+}
+
+// Scan part 2: exclusive prefix of nnz into c_rpt (binning.cuh:219-224),
+// c_rpt[m] = nnz(C), and the scatter of rows into the NUM bin lists.  The
+// prefix of a tile is the sum of the earlier tiles of its product, which
+// every workgroup reduces itself (no inter-workgroup waiting).
+template <typename V>
+__global__ __launch_bounds__(kRowThreads) void k_scan_apply(ProbSet<V> ps, BinTable num, DevMeta* meta,
+                                                            const long long* __restrict__ tile_sums,
+                                                            int2* __restrict__ items,
+                                                            long long* __restrict__ gpool_off,
+                                                            long long* __restrict__ inst_nnz, int after_fail) {
+    if (!after_fail && meta->fail_count > 0) return;
+    constexpr int kPer = kTileRows / kRowThreads;  // consecutive rows per thread
+    constexpr int kWaves = kRowThreads / kWave;
+    __shared__ long long s_wave[kWaves];
+    __shared__ long long s_prefix;
+    __shared__ int s_cnt[kMaxBins];
+    __shared__ int s_base[kMaxBins];
+    __shared__ int s_flags;
+
+    const Tile t = ps.tile(blockIdx.x);
+    const Prob<V> P = ps.get(t.inst);
+    if (threadIdx.x < kMaxBins) s_cnt[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_flags = 0;
+
+    // prefix over earlier tiles of this product
+    long long pre = 0;
+    for (int i = t.first_tile + threadIdx.x; i < static_cast<int>(blockIdx.x); i += kRowThreads)
+        pre += tile_sums[i];
+    pre = wave_sum64(pre);
+    if (lane_id() == 0) s_wave[threadIdx.x / kWave] = pre;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long s = 0;
+        for (int w = 0; w < kWaves; ++w) s += s_wave[w];
+        s_prefix = s;
+    }
+    __syncthreads();
+    const long long tile_base = s_prefix;
+
+    // thread-local values (consecutive rows)
+    int v[kPer];
+    int bin[kPer];
+    int rank[kPer];
+    long long local = 0;
+    const int r0 = threadIdx.x * kPer;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int r = r0 + j;
+        v[j] = r < t.rows ? P.c_rpt[t.row0 + r] : 0;
+        local += v[j];
+        bin[j] = -1;
+        if (v[j] > 0) {
+            bin[j] = find_bin(num, P.nip[t.row0 + r], v[j]);
+            rank[j] = atomicAdd(&s_cnt[bin[j]], 1);
+        }
+    }
+    // block exclusive scan of `local`
+    long long incl = local;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const long long y = __shfl_up(incl, o);
+        if (lane_id() >= o) incl += y;
+    }
+    __syncthreads();  // s_wave reuse
+    if (lane_id() == kWave - 1) s_wave[threadIdx.x / kWave] = incl;
+    __syncthreads();
+    long long wave_base = 0;
+    for (int w = 0; w < static_cast<int>(threadIdx.x / kWave); ++w) wave_base += s_wave[w];
+    long long run = tile_base + wave_base + incl - local;
+
+    int flags = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int r = r0 + j;
+        if (r < t.rows) {
+            if (run > INT_MAX) flags |= kFlagNnzOverflow;
+            P.c_rpt[t.row0 + r] = static_cast<int>(run);
+        }
+        run += v[j];
+    }
+    // last row of the product: c_rpt[m] = nnz(C)
+    if (t.rows > 0 && r0 < t.rows && r0 + kPer >= t.rows && t.row0 + t.rows == P.m) {
+        if (run > INT_MAX) flags |= kFlagNnzOverflow;
+        P.c_rpt[P.m] = static_cast<int>(run > INT_MAX ? INT_MAX : run);
+        inst_nnz[t.inst] = run;
+        if (ps.n_inst == 1) meta->nnz_total = run;
+    }
+    if (flags) atomicOr(&s_flags, flags);
+    __syncthreads();
+
+    if (threadIdx.x < num.n && s_cnt[threadIdx.x] > 0) {
+        int off = 0;
+        for (int b = 0; b < static_cast<int>(threadIdx.x); ++b) off += meta->num_count[b];
+        s_base[threadIdx.x] = off + atomicAdd(&meta->num_cursor[threadIdx.x], s_cnt[threadIdx.x]);
+    }
+    if (threadIdx.x == 0 && s_flags) atomicOr(&meta->flags, s_flags);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        if (bin[j] < 0) continue;
+        const int pos = s_base[bin[j]] + rank[j];
+        items[pos] = make_int2(t.inst, t.row0 + r0 + j);
+        if (num.variant[bin[j]] == kNumGlobalVariant)
+            gpool_off[pos] = static_cast<long long>(
+                atomicAdd(&meta->gnum_cursor, static_cast<unsigned long long>(global_table_slots(v[j]))));
+    }
+}
+
+// Total NIP of A . B for the FLOP metric (nip.cuh:19-49), one thread per row.
+__global__ __launch_bounds__(kRowThreads) void k_total_nip(const int32_t* __restrict__ a_rpt,
+                                                           const int32_t* __restrict__ a_col,
+                                                           const int32_t* __restrict__ b_rpt, int m,
+                                                           unsigned long long* out) {
+    const int row = blockIdx.x * kRowThreads + threadIdx.x;
+    long long s = 0;
+    if (row < m)
+        for (int p = a_rpt[row]; p < a_rpt[row + 1]; ++p) s += b_rpt[a_col[p] + 1] - b_rpt[a_col[p]];
+    s = wave_sum64(s);
+    if (lane_id() == 0 && s) atomicAdd(out, static_cast<unsigned long long>(s));
+}
+
+// ----------------------------------------------------------------- launchers
+
+void launch_total_nip(const int32_t* a_rpt, const int32_t* a_col, const int32_t* b_rpt, int m,
+                      unsigned long long* out, hipStream_t s) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_total_nip, dim3((m + kRowThreads - 1) / kRowThreads), dim3(kRowThreads), 0, s, a_rpt,
+                       a_col, b_rpt, m, out);
+}
+
+void launch_clear(DevMeta* meta, int from_word, hipStream_t s) {
+    hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, s, meta, from_word);
+}
+
+template <typename V>
+void launch_nip(const ProbSet<V>& ps, const BinTable& sym, DevMeta* meta, hipStream_t s) {
+    hipLaunchKernelGGL(k_nip<V>, dim3(ps.n_tiles), dim3(kRowThreads), 0, s, ps, sym, meta);
+}
+
+template <typename V>
+void launch_scatter(const ProbSet<V>& ps, const BinTable& sym, DevMeta* meta, int2* items, hipStream_t s) {
+    hipLaunchKernelGGL(k_scatter<V>, dim3(ps.n_tiles), dim3(kRowThreads), 0, s, ps, sym, meta, items);
+}
+
+template <typename V>
+void launch_scan_count(const ProbSet<V>& ps, const BinTable& num, DevMeta* meta, long long* tile_sums,
+                       int after_fail, hipStream_t s) {
+    hipLaunchKernelGGL(k_scan_count<V>, dim3(ps.n_tiles), dim3(kRowThreads), 0, s, ps, num, meta, tile_sums,
+                       after_fail);
+}
+
+template <typename V>
+void launch_scan_apply(const ProbSet<V>& ps, const BinTable& num, DevMeta* meta, const long long* tile_sums,
+                       int2* items, long long* gpool_off, long long* inst_nnz, int after_fail, hipStream_t s) {
+    hipLaunchKernelGGL(k_scan_apply<V>, dim3(ps.n_tiles), dim3(kRowThreads), 0, s, ps, num, meta, tile_sums,
+                       items, gpool_off, inst_nnz, after_fail);
+}
+
+#define SPG_INSTANTIATE_ROWS(V)                                                                        \
+    template void launch_nip<V>(const ProbSet<V>&, const BinTable&, DevMeta*, hipStream_t);           \
+    template void launch_scatter<V>(const ProbSet<V>&, const BinTable&, DevMeta*, int2*, hipStream_t); \
+    template void launch_scan_count<V>(const ProbSet<V>&, const BinTable&, DevMeta*, long long*, int,  \
+                                       hipStream_t);                                                   \
+    template void launch_scan_apply<V>(const ProbSet<V>&, const BinTable&, DevMeta*, const long long*, \
+                                       int2*, long long*, long long*, int, hipStream_t);
+
+SPG_INSTANTIATE_ROWS(float)
+SPG_INSTANTIATE_ROWS(double)
+
+}  // namespace spg

@@ -1,0 +1,228 @@
+// Shared definitions of the gfx950 SpGEMM kernels and their host driver.
+//
+// Pipeline of one multiply (reference phases in brackets, paths relative to
+// src/libproposal/include/proposal/):
+//   k_clear      zero the device counters                  [setup.cuh:146-150]
+//   k_nip        NIP per row, SYM bin histogram, max NIP   [setup.cuh:29-95, binning.cuh:39-72]
+//   k_scatter    rows -> SYM bin lists                     [binning.cuh:76-120]
+//   k_sym_multi  exact nnz per row by LDS hash tables      [sym1.cuh:28-167]
+//   k_sym_max    largest LDS table with overflow detection [sym1.cuh:169-245]
+//   k_sym_global global-memory tables for overflowed rows  [sym1.cuh:247-306]
+//   k_scan_count / k_scan_apply
+//                exclusive scan nnz -> row_ptr, NUM bins   [binning.cuh:208-243]
+//   k_symnum_*   sorted columns and values of each row in one pass: the
+//                hash fill and in-LDS sort of sym2.cuh:27-288 followed, on
+//                the same LDS-resident sorted column list, by the
+//                binary-search accumulation of num.cuh:28-298.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstddef>
+#include <cstdint>
+
+namespace spg {
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kEmpty = -1;         // empty hash slot (columns are >= 0)
+constexpr int kMaxBins = 16;
+constexpr int kTileRows = 1024;    // rows per tile of the row-parallel passes
+constexpr int kRowThreads = 256;   // threads of the row-parallel passes
+constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups dealt round-robin
+constexpr int kMultiThreads = 256; // workgroup size of the multi-bin kernels
+
+// One product C = A . B.  `nip` is per-row scratch; c_rpt first holds nnz
+// per row and then the row pointers (the reference reuses C.rpt the same
+// way, proposal.cuh:40-56).
+template <typename V>
+struct Prob {
+    const int32_t* a_rpt;
+    const int32_t* a_col;
+    const V* a_val;
+    const int32_t* b_rpt;
+    const int32_t* b_col;
+    const V* b_val;
+    int32_t* c_rpt;
+    int32_t* c_col;
+    V* c_val;
+    int32_t* nip;
+    int32_t m;
+    int32_t n;
+};
+
+// Tile of a row-parallel pass: instance, first row, row count, index of the
+// instance's first tile.
+struct Tile {
+    int32_t inst, row0, rows, first_tile;
+};
+
+// All products of one pass.  A single product travels in kernel arguments;
+// a batch is read from device memory.
+template <typename V>
+struct ProbSet {
+    Prob<V> p0;
+    const Prob<V>* many;
+    const Tile* tiles;
+    int32_t n_inst;
+    int32_t n_tiles;
+
+    __device__ __forceinline__ Prob<V> get(int i) const { return n_inst == 1 ? p0 : many[i]; }
+    __device__ __forceinline__ Tile tile(int t) const {
+        if (n_inst == 1) {
+            const int r0 = t * kTileRows;
+            return Tile{0, r0, min(kTileRows, p0.m - r0), 0};
+        }
+        return tiles[t];
+    }
+};
+
+// Runtime bin table of one phase: row -> first bin whose caps admit it.
+struct BinTable {
+    int32_t n;
+    int32_t variant[kMaxBins];
+    int32_t rows[kMaxBins];  // rows per workgroup unit of the bin's variant
+    int32_t nip_cap[kMaxBins];
+    int32_t nnz_cap[kMaxBins];
+};
+
+__host__ __device__ __forceinline__ int find_bin(const BinTable& t, int nip, int nnz) {
+    for (int b = 0; b < t.n; ++b)
+        if (nip <= t.nip_cap[b] && nnz <= t.nnz_cap[b]) return b;
+    return t.n - 1;
+}
+
+enum MetaFlags : int32_t {
+    kFlagNipOverflow = 1,   // a row's NIP >= 2^31
+    kFlagNnzOverflow = 2,   // nnz(C) >= 2^31
+    kFlagMismatch = 4,      // fused pass found a different nnz than the count pass
+};
+
+// Device counters of one multiply (zeroed by k_clear).  The second group
+// belongs to the scan / fused-bin stage and is cleared again on its own
+// when overflowed rows force that stage to run twice.
+struct DevMeta {
+    int32_t sym_count[kMaxBins];
+    int32_t sym_cursor[kMaxBins];
+    int32_t fail_count;
+    int32_t flags;
+    int32_t max_nip;
+    int32_t pad0;
+    unsigned long long total_nip;
+    unsigned long long fail_pool;    // int32 slots of the global tables of overflowed rows
+    // ---- scan stage
+    int32_t num_count[kMaxBins];
+    int32_t num_cursor[kMaxBins];
+    int32_t max_nnz;
+    int32_t pad1;
+    unsigned long long gnum_pool;    // int32 slots needed by the global fused bin
+    unsigned long long gnum_cursor;  // slots handed out so far
+    long long nnz_total;             // nnz(C) of a single product
+};
+constexpr int kMetaScanWord = static_cast<int>(offsetof(DevMeta, num_count) / sizeof(int32_t));
+
+// Launch description of a multi-bin kernel: the bins it serves, their item
+// ranges and the cumulative count of work units (one unit = one workgroup's
+// rows).  For the symbolic pass counts are read on the device instead.
+struct MultiArgs {
+    int32_t n;                  // bins served
+    int32_t bin[kMaxBins];      // bin index in the phase table
+    int32_t variant[kMaxBins];  // compiled variant
+    int32_t rows[kMaxBins];     // rows per unit
+    int32_t item_off[kMaxBins];
+    int32_t count[kMaxBins];
+    int32_t unit_start[kMaxBins + 1];
+};
+
+// ---------------------------------------------------------------- device helpers
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+template <int T>
+__device__ __forceinline__ uint32_t hash_slot(int key) {
+    static_assert((T & (T - 1)) == 0 && T >= 2, "table size must be a power of two");
+    constexpr int kBits = __builtin_ctz(T);
+    return (static_cast<uint32_t>(key) * 2654435761u) >> (32 - kBits);
+}
+
+__device__ __forceinline__ uint32_t hash_slot_rt(int key, int bits) {
+    return (static_cast<uint32_t>(key) * 2654435761u) >> (32 - bits);
+}
+
+__device__ __forceinline__ int lds_load(const int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Open-addressing insert with linear probing into an LDS table of T slots.
+// Returns 1 when the key was not present.  The table never fills: callers
+// size T above the number of distinct keys of the row.
+template <int T>
+__device__ __forceinline__ int table_insert(int* tab, int key) {
+    uint32_t h = hash_slot<T>(key);
+    while (true) {
+        const int cur = lds_load(tab + h);
+        if (cur == key) return 0;
+        if (cur == kEmpty) {
+            const int old = atomicCAS(tab + h, kEmpty, key);
+            if (old == kEmpty) return 1;
+            if (old == key) return 0;
+        }
+        h = (h + 1) & (T - 1);
+    }
+}
+
+// Sum over the L consecutive lanes of a row group (L a power of two <= 64).
+template <int L>
+__device__ __forceinline__ int group_sum(int v) {
+    if constexpr (L > 1) {
+#pragma unroll
+        for (int o = L / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, L);
+    }
+    return v;
+}
+
+__device__ __forceinline__ long long wave_sum64(long long v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Branch-free lower bound in a sorted LDS list of P entries (P a power of
+// two, padded with INT_MAX).  The key is known to be present.
+template <int P>
+__device__ __forceinline__ int lower_bound_pow2(const int* cols, int key) {
+    int pos = 0;
+#pragma unroll
+    for (int step = P >> 1; step > 0; step >>= 1)
+        if (cols[pos + step - 1] < key) pos += step;
+    return pos;
+}
+
+template <typename V>
+__device__ __forceinline__ void lds_add(V* p, V v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// XCD-aware share of a unit range [0, total) for this workgroup: each XCD
+// takes one contiguous slice so that neighbouring rows (which read the same
+// rows of B) meet in the same L2.  gridDim.x is a multiple of kXcds.
+struct UnitRange {
+    int first, end, stride;
+};
+__device__ __forceinline__ UnitRange xcd_units(int total) {
+    const int xcd = blockIdx.x % kXcds;
+    const int local = blockIdx.x / kXcds;
+    const int per_xcd = gridDim.x / kXcds;
+    const int slice = (total + kXcds - 1) / kXcds;
+    const int lo = xcd * slice;
+    const int hi = min(total, lo + slice);
+    return UnitRange{lo + local, hi, per_xcd};
+}
+
+}  // namespace spg

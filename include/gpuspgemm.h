@@ -1,0 +1,130 @@
+/*
+ * gpuspgemm.h -- C ABI of the MI355X (gfx950) SpGEMM library.
+ *
+ * Every entry point returns an spg_status and never throws.  Matrices are
+ * 0-based CSR with int32 indices, as utils::CSR in the reference
+ * (src/libutils/include/utils/csr.cuh:23-31); `val` is float or double per
+ * spg_dtype.  Device pointers are plain HIP device addresses.
+ *
+ * Reference interfaces each entry point stands in for (paths relative to
+ * the GPUSpGEMM source tree):
+ *
+ *   spg_multiply        proposal::proposal<T>(A, B) -> C
+ *                       src/libproposal/include/proposal/proposal.cuh:85-124
+ *   spg_csr_free        utils::CSR<T,Device>::~CSR / release()
+ *                       src/libutils/include/utils/csr.cuh:114-117, 241-252
+ *   spg_get_nip         utils::get_nip(a, b) -> int64
+ *                       src/libutils/include/utils/nip.cuh:52-78
+ *   spg_malloc/spg_free utils::malloc<Device>/free<Device>
+ *                       src/libutils/include/utils/runtime.cuh:11-20, 58-66
+ *   spg_memcpy          utils::memcpy (host<->device, cudaMemcpyDefault)
+ *                       src/libutils/include/utils/runtime.cuh:45-48; used by
+ *                       CSR::to<Location>() csr.cuh:193-203
+ *   spg_device_sync     utils::device_sync  runtime.cuh:98-101
+ *   spg_status_str /    utils::handle_cuda_error message text
+ *   spg_last_error      src/libutils/include/utils/errors.cuh:11-22
+ *
+ * Library-specific additions (no reference counterpart; needed by the
+ * drop-in harness, the benchmark and the tuner):
+ *   spg_init, spg_device_info, spg_set_tuning, spg_get_tuning,
+ *   spg_set_profiling, spg_profile_count, spg_profile_get,
+ *   spg_multiply_batched.
+ */
+#ifndef GPUSPGEMM_H
+#define GPUSPGEMM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum spg_status {
+    SPG_OK = 0,
+    SPG_ERR_INVALID_ARGUMENT = 1, /* null pointer, A.n != B.m, negative sizes */
+    SPG_ERR_OUT_OF_MEMORY = 2,    /* device allocation failed */
+    SPG_ERR_HIP = 3,              /* a HIP runtime call failed; see spg_last_error */
+    SPG_ERR_OVERFLOW = 4,         /* nnz(C) or a row's product count >= 2^31 */
+    SPG_ERR_UNSUPPORTED_DEVICE = 5, /* not a gfx950 device */
+    SPG_ERR_INTERNAL = 6          /* a consistency check inside the library failed */
+} spg_status;
+
+typedef enum spg_dtype { SPG_F32 = 0, SPG_F64 = 1 } spg_dtype;
+
+/* Same members and order as utils::CSR (csr.cuh:25-30). */
+typedef struct spg_csr {
+    int32_t nnz;
+    int32_t m;
+    int32_t n;
+    int32_t *rpt; /* [m + 1] */
+    int32_t *col; /* [nnz]   */
+    void *val;    /* [nnz] float or double */
+} spg_csr;
+
+/* Initialise the library on the current HIP device (idempotent; called by
+   every entry point on first use). */
+spg_status spg_init(void);
+
+/* C = A . B on the current device.  A and B are borrowed device matrices
+   (B's rows sorted ascending, indices in range).  On success *C holds
+   library-allocated device arrays (release with spg_csr_free) and the
+   result is complete when the call returns, as the reference's
+   synchronous proposal(). */
+spg_status spg_multiply(const spg_csr *A, const spg_csr *B, spg_csr *C, spg_dtype dtype);
+
+/* Many independent products C_i = A_i . B_i in one pass of the kernels
+   (rows of all instances are binned together).  Arrays have `count`
+   entries; each C[i] is allocated like spg_multiply's. */
+spg_status spg_multiply_batched(int32_t count, const spg_csr *A, const spg_csr *B,
+                                spg_csr *C, spg_dtype dtype);
+
+/* Free the device arrays of a library-allocated matrix and zero it. */
+spg_status spg_csr_free(spg_csr *C);
+
+/* Total number of intermediate products sum_i sum_{k in A(i,:)} nnz(B(k,:)). */
+spg_status spg_get_nip(const spg_csr *A, const spg_csr *B, int64_t *nip);
+
+/* Device memory helpers for hosts that do not link HIP themselves. */
+spg_status spg_malloc(void **ptr, size_t bytes);
+spg_status spg_free(void *ptr);
+spg_status spg_memcpy(void *dst, const void *src, size_t bytes); /* any direction */
+spg_status spg_device_sync(void);
+
+const char *spg_status_str(spg_status status);
+/* Message of the last failure on this thread ("" if none). */
+const char *spg_last_error(void);
+
+/* Device facts used by the tuner and the benchmark. */
+typedef struct spg_device_info {
+    char arch[64];           /* e.g. "gfx950" */
+    int32_t compute_units;
+    int32_t lds_per_block;   /* bytes */
+    int32_t clock_khz;
+    int64_t global_mem_bytes;
+} spg_device_info;
+spg_status spg_device_info_get(spg_device_info *info);
+
+/* Bin tables.  phase 0 = symbolic count (bins by NIP), phase 1 = the fused
+   sorted-columns + numeric pass (bins by nnz and NIP).  Each bin names a
+   compiled kernel variant and the largest NIP / nnz it accepts; rows go to
+   the first bin that accepts them.  Text format, one bin per line:
+   "<variant> <nip_cap> <nnz_cap>". */
+spg_status spg_set_tuning(int32_t phase, const char *table);
+spg_status spg_get_tuning(int32_t phase, char *buf, size_t buf_len);
+/* Names of the compiled variants of a phase, one per line. */
+spg_status spg_get_variants(int32_t phase, char *buf, size_t buf_len);
+
+/* Per-kernel timing with HIP events on the library stream (off by
+   default).  After a multiply, spg_profile_count() records are available:
+   kernel name, rows handled, launch grid and duration in milliseconds. */
+spg_status spg_set_profiling(int32_t enabled);
+int32_t spg_profile_count(void);
+spg_status spg_profile_get(int32_t index, char *name, size_t name_len, int64_t *rows,
+                           int32_t *grid, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUSPGEMM_H */
