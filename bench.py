@@ -1,0 +1,244 @@
+#!/usr/bin/env python3
+"""Benchmark of C = A.A on one MI355X: GFLOP/s (2 x intermediate products
+per second, the reference's metric, src/apps/common.hpp:186-204) and HBM
+GB/s of algorithmic bytes.
+
+A step is one full multiply with A and B (two device copies, as the
+reference's app makes, src/apps/common.hpp:170-171) resident in HBM: the
+wall clock spans the whole call, including the allocation of C and the
+final synchronisation, as src/apps/proposal.cu:28-35 times it; freeing C is
+outside the timed region, as in the reference.
+
+Default workload: the power-law matrix (1,000,000 rows, Pareto(2) row
+lengths), FP64.  The stencil (96^3, 27-point) is measured beside it and
+reported under "secondary".
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 10] [--workload powerlaw]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E, published
+METRIC = "SpGEMM GFLOP/s (2×intermediate products) + HBM GB/s, FP64; served p50/p99"
+
+
+def make_workload(name: str):
+    from gpuspgemm_amd import generators as gen
+    if name == "powerlaw":
+        return gen.powerlaw(), {"workload": "powerlaw", "rows": 1_000_000, "row_length": "Pareto(2), mean ~3",
+                                "columns": "70% within +-64 of the diagonal, 30% uniform", "seed": 1}
+    if name == "stencil":
+        return gen.stencil27(), {"workload": "stencil27", "grid": "96^3", "rows": 96 ** 3, "seed": 2}
+    if name == "uniform8":
+        return gen.uniform8(), {"workload": "uniform8", "rows": 4096, "per_row": 8, "seed": 0}
+    raise SystemExit(f"unknown workload {name}")
+
+
+def csr_bytes(m: int, nnz: int, s: int) -> int:
+    return 4 * (m + 1) + (4 + s) * nnz
+
+
+def row_nip(A, B) -> np.ndarray:
+    blen = np.diff(B.rpt.astype(np.int64))
+    per_entry = blen[A.col]
+    cs = np.concatenate([[0], np.cumsum(per_entry)])
+    return cs[A.rpt[1:].astype(np.int64)] - cs[A.rpt[:-1].astype(np.int64)]
+
+
+def classify(table: str, nip: np.ndarray, nnz: np.ndarray) -> tuple[np.ndarray, list[int]]:
+    """Bin of each row under a bin table (first bin whose caps admit it)."""
+    bins = [tuple(int(x) for x in line.split()) for line in table.strip().splitlines()]
+    out = np.full(nip.shape, len(bins) - 1, dtype=np.int64)
+    todo = np.ones(nip.shape, dtype=bool)
+    for b, (_, nip_cap, nnz_cap) in enumerate(bins):
+        hit = todo & (nip <= nip_cap) & (nnz <= nnz_cap)
+        out[hit] = b
+        todo &= ~hit
+    return out, [v for v, _, _ in bins]
+
+
+def kernel_bytes(name: str, A, B, C_rpt: np.ndarray, s: int, sym_table: str, num_table: str) -> int:
+    """Algorithmic (compulsory) bytes of one launch of kernel `name`: the A
+    rows it handles (row pointers, columns, values when it reads them),
+    every B row those rows reference, read once, and what it writes."""
+    nip = row_nip(A, B)
+    nnz_c = np.diff(C_rpt.astype(np.int64))
+    arow = np.diff(A.rpt.astype(np.int64))
+    if name.startswith("symnum_v"):
+        variant = int(name[len("symnum_v"):])
+        b, variants = classify(num_table, nip, nnz_c)
+        sel = (nnz_c > 0) & np.isin(b, [i for i, v in enumerate(variants) if v == variant])
+        val_a = val_b = s
+        write = (4 + s) * int(nnz_c[sel].sum())
+    elif name == "sym_multi":
+        b, variants = classify(sym_table, nip, np.zeros_like(nip))
+        sel = (nip > 0) & (b < len(variants) - 1)
+        val_a = val_b = 0
+        write = 4 * int(sel.sum())
+    else:
+        return 0
+    rows = np.nonzero(sel)[0]
+    a_bytes = 8 * rows.size + (4 + val_a) * int(arow[rows].sum())
+    mask = np.repeat(sel, arow)
+    used_b = np.unique(A.col[mask])
+    blen = np.diff(B.rpt.astype(np.int64))
+    b_bytes = 8 * used_b.size + (4 + val_b) * int(blen[used_b].sum())
+    return a_bytes + b_bytes + write
+
+
+def traffic_record(workload: str, dtype: str, kernel: str):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fp:
+            data = json.load(fp)
+        return data.get(f"{workload}/{dtype}/{kernel}")
+    except (OSError, ValueError):
+        return None
+
+
+def run_gpu(A, steps: int, warmup: int, pause_ms: float, dtype) -> dict:
+    import gpuspgemm_amd as g
+    g.load_library()
+    Ah = A.astype(dtype)
+    dA, dB = Ah.to_device(), Ah.to_device()
+    nip = g.get_nip(dA, dB)
+    for _ in range(warmup):
+        if pause_ms:
+            time.sleep(pause_ms / 1e3)
+        g.proposal(dA, dB).free()
+    times = []
+    kernel_ms: dict[str, list[float]] = {}
+    with g.profile() as prof:
+        for _ in range(steps):
+            if pause_ms:
+                time.sleep(pause_ms / 1e3)
+            t0 = time.perf_counter()
+            C = g.proposal(dA, dB)
+            t1 = time.perf_counter()
+            for r in prof.collect():
+                kernel_ms.setdefault(r["name"], []).append(r["ms"])
+            times.append(t1 - t0)
+            if _ == steps - 1:
+                c_rpt = C.to_host().rpt
+                c_nnz = C.nnz
+            C.free()
+    sym_table, num_table = g.get_tuning(0), g.get_tuning(1)
+    dA.free()
+    dB.free()
+    return {"times": np.array(times), "nip": nip, "c_rpt": c_rpt, "c_nnz": c_nnz, "kernel_ms": kernel_ms,
+            "sym_table": sym_table, "num_table": num_table, "info": g.device_info()}
+
+
+def summarize(A, res: dict, dtype_name: str, workload: str, cfg: dict) -> dict:
+    s = 8 if dtype_name == "f64" else 4
+    t = res["times"]
+    mean = float(t.mean())
+    flop = 2.0 * res["nip"]
+    b_alg = 2 * csr_bytes(A.m, A.nnz, s) + csr_bytes(A.m, res["c_nnz"], s)
+    # dominant kernel by total time over the timed steps
+    totals = {k: float(np.sum(v)) for k, v in res["kernel_ms"].items()}
+    dom = max(totals, key=totals.get)
+    dom_avg_ms = float(np.mean(res["kernel_ms"][dom]))
+    dom_bytes = kernel_bytes(dom, A, A, res["c_rpt"], s, res["sym_table"], res["num_table"])
+    achieved = dom_bytes / (dom_avg_ms * 1e-3) / 1e9 if dom_bytes else None
+    traffic = traffic_record(workload, dtype_name, dom)
+    return {
+        "value": flop / mean / 1e9,
+        "ms_per_step": mean * 1e3,
+        "p50_ms": float(np.percentile(t, 50)) * 1e3,
+        "p99_ms": float(np.percentile(t, 99)) * 1e3,
+        "hbm_gbps": b_alg / mean / 1e9,
+        "hbm_frac": b_alg / mean / 1e9 / HBM_PEAK_GBPS,
+        "nip": int(res["nip"]),
+        "nnz_c": int(res["c_nnz"]),
+        "alg_bytes": int(b_alg),
+        "kernels_ms": {k: round(float(np.mean(v)), 4) for k, v in sorted(res["kernel_ms"].items())},
+        "roofline": {
+            "bound": "hbm", "kernel": dom,
+            "achieved": round(achieved, 1) if achieved else None,
+            "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4) if achieved else None,
+            "traffic": traffic, "alg_bytes_per_launch": int(dom_bytes), "avg_launch_ms": round(dom_avg_ms, 4),
+        },
+        "config": cfg,
+    }
+
+
+def cpu_baseline(A, nip: int, budget_s: float) -> dict:
+    import oracle
+    threads = oracle.hw_threads()
+    oracle.spgemm(A, A, threads=threads)  # warm caches / page in
+    runs, t_total = 0, 0.0
+    while runs < 3 or (t_total < budget_s and runs < 50):
+        t0 = time.perf_counter()
+        oracle.spgemm(A, A, threads=threads)
+        t_total += time.perf_counter() - t0
+        runs += 1
+    return {"value": 2.0 * nip * runs / t_total / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": "port",
+            "sample": f"full product, {runs} runs of the oracle's row-parallel Gustavson "
+                      f"({threads} threads), {t_total:.1f} s"}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--pause", type=float, default=0.0, help="ms of sleep before each call (reference: 50)")
+    ap.add_argument("--workload", default="powerlaw", choices=["powerlaw", "stencil", "uniform8"])
+    ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
+    ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the library is single-GPU, "
+              "see DESIGN.md for how it would shard)", file=sys.stderr)
+        return 2
+    dtype = np.float64 if args.dtype == "f64" else np.float32
+
+    A, cfg = make_workload(args.workload)
+    res = run_gpu(A, args.steps, args.warmup, args.pause, dtype)
+    out = summarize(A, res, args.dtype, args.workload, cfg)
+    line = {
+        "metric": METRIC, "value": round(out["value"], 3), "unit": "GFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(out["ms_per_step"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": args.dtype,
+        "data": "synthetic", "config": cfg,
+        "hbm_gbps": round(out["hbm_gbps"], 1), "hbm_frac": round(out["hbm_frac"], 4),
+        "p50_ms": round(out["p50_ms"], 4), "p99_ms": round(out["p99_ms"], 4),
+        "nip": out["nip"], "nnz_c": out["nnz_c"], "alg_bytes": out["alg_bytes"],
+        "kernels_ms": out["kernels_ms"], "roofline": out["roofline"],
+        "device": res["info"],
+    }
+    if not args.no_secondary and args.workload == "powerlaw":
+        S, scfg = make_workload("stencil")
+        sres = run_gpu(S, args.steps, min(args.warmup, 3), args.pause, dtype)
+        sout = summarize(S, sres, args.dtype, "stencil", scfg)
+        line["secondary"] = {
+            "config": scfg, "value": round(sout["value"], 3), "unit": "GFLOP/s",
+            "ms_per_step": round(sout["ms_per_step"], 4), "hbm_gbps": round(sout["hbm_gbps"], 1),
+            "hbm_frac": round(sout["hbm_frac"], 4), "kernels_ms": sout["kernels_ms"],
+            "roofline": sout["roofline"], "nip": sout["nip"], "nnz_c": sout["nnz_c"],
+        }
+        del S
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(A.astype(dtype), out["nip"], args.cpu_budget)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -78,7 +78,11 @@ struct Context {
     BinTable sym{};
     BinTable num{};
     int occ_sym_multi = 1, occ_sym_max = 1;
-    int occ_num_multi[2] = {1, 1}, occ_num_big[2] = {1, 1};
+    int occ_num[2][kNumLdsVariants] = {};
+    static constexpr int kStreams = 4;  // fused-pass launches are spread over these
+    hipStream_t aux[kStreams - 1] = {};
+    hipEvent_t ev_ready = nullptr;
+    hipEvent_t ev_done[kStreams - 1] = {};
     char* scratch = nullptr;
     size_t scratch_cap = 0;
     DevMeta* h_meta = nullptr;
@@ -97,23 +101,26 @@ Context g_ctx[kMaxDevices];
 // ---------------------------------------------------------------- bin tables
 
 void default_tables(BinTable& sym, BinTable& num) {
-    // Symbolic count: bins by NIP; a table of T slots takes rows of NIP < T/2.
-    const int sym_caps[] = {8, 16, 32, 64, 128, 256, 1024, 4096, INT_MAX};
+    // Symbolic count: bins by NIP; a row's distinct columns (<= NIP) stay
+    // below its table size, at most 3/4 full.
+    const int sym_caps[] = {8, 16, 32, 64, 128, 256, 768, 2048, 4096, INT_MAX};
     sym = BinTable{};
-    sym.n = 9;
+    sym.n = static_cast<int>(sizeof(sym_caps) / sizeof(sym_caps[0]));
     for (int b = 0; b < sym.n; ++b) {
         sym.variant[b] = b;
         sym.rows[b] = kSymVariants[b].r;
         sym.nip_cap[b] = sym_caps[b];
         sym.nnz_cap[b] = INT_MAX;
     }
-    // Fused pass: bins by (nnz, NIP).  nnz_cap never exceeds the variant's P.
+    // Fused pass: bins by (nnz, NIP).  nnz_cap never exceeds the variant's
+    // capacity; the NIP caps keep rows on variants whose lanes can hold
+    // their products in registers.
     struct Row {
         int variant, nip, nnz;
     };
-    const Row rows[] = {{0, 16, 8},        {1, 32, 16},        {2, 64, 32},         {3, 128, 64},
-                        {4, 256, 128},     {5, 2048, 128},     {6, 4096, 512},      {7, INT_MAX, 1024},
-                        {8, INT_MAX, 2048}, {9, INT_MAX, 8192}, {kNumGlobalVariant, INT_MAX, INT_MAX}};
+    const Row rows[] = {{0, 16, 8},          {1, 32, 16},        {2, 64, 32},         {3, 128, 64},
+                        {4, 256, 128},       {5, 768, 128},      {6, 4096, 512},      {7, INT_MAX, 1024},
+                        {8, INT_MAX, 2048},  {9, INT_MAX, 8192}, {kNumGlobalVariant, INT_MAX, INT_MAX}};
     num = BinTable{};
     num.n = static_cast<int>(sizeof(rows) / sizeof(rows[0]));
     for (int b = 0; b < num.n; ++b) {
@@ -132,12 +139,12 @@ spg_status check_sym_table(const BinTable& t) {
     for (int b = 0; b + 1 < t.n; ++b) {
         const int v = t.variant[b];
         if (v < 0 || v >= kSymMultiVariants)
-            return fail(SPG_ERR_INVALID_ARGUMENT, "symbolic table: bins before the last must use variants 0-7");
+            return fail(SPG_ERR_INVALID_ARGUMENT, "symbolic table: bins before the last must use the multi-bin variants");
         if (t.nip_cap[b] >= kSymVariants[v].t || t.nip_cap[b] < 1)
             return fail(SPG_ERR_INVALID_ARGUMENT, "symbolic table: NIP cap must be below the variant's table size");
     }
     if (t.variant[t.n - 1] != kSymMaxVariant || t.nip_cap[t.n - 1] != INT_MAX)
-        return fail(SPG_ERR_INVALID_ARGUMENT, "symbolic table: last bin must be variant 8 with cap INT_MAX");
+        return fail(SPG_ERR_INVALID_ARGUMENT, "symbolic table: last bin must be the largest-table variant with cap INT_MAX");
     return SPG_OK;
 }
 
@@ -173,7 +180,8 @@ spg_status ensure_context(Context*& out) {
     c.lds = static_cast<int>(prop.sharedMemPerBlock);
     c.clock_khz = prop.clockRate;
     c.global_mem = prop.totalGlobalMem;
-    if (c.lds < kNumBigLds + 1024 || c.lds < (kSymVariants[kSymMaxVariant].t * 4 + 64))
+    if (c.lds < kNumBigLds + 1024 ||
+        c.lds < kSymVariants[kSymMaxVariant].t * 4 + 8 * kSymVariants[kSymMaxVariant].ec + 1024)
         return fail(SPG_ERR_UNSUPPORTED_DEVICE, "device offers less LDS per workgroup than the kernels need");
     SPG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     // Keep freed blocks in the stream-ordered pool: C is allocated per call.
@@ -188,10 +196,15 @@ spg_status ensure_context(Context*& out) {
     default_tables(c.sym, c.num);
     c.occ_sym_multi = occupancy_sym_multi<double>();
     c.occ_sym_max = occupancy_sym_max<double>();
-    c.occ_num_multi[0] = occupancy_symnum_multi<float>();
-    c.occ_num_multi[1] = occupancy_symnum_multi<double>();
-    c.occ_num_big[0] = occupancy_symnum_big<float>();
-    c.occ_num_big[1] = occupancy_symnum_big<double>();
+    for (int v = 0; v < kNumLdsVariants; ++v) {
+        c.occ_num[0][v] = occupancy_symnum_bin<float>(v);
+        c.occ_num[1][v] = occupancy_symnum_bin<double>(v);
+    }
+    for (int i = 0; i < Context::kStreams - 1; ++i) {
+        SPG_HIP(hipStreamCreateWithFlags(&c.aux[i], hipStreamNonBlocking));
+        SPG_HIP(hipEventCreateWithFlags(&c.ev_done[i], hipEventDisableTiming));
+    }
+    SPG_HIP(hipEventCreateWithFlags(&c.ev_ready, hipEventDisableTiming));
     c.ready = true;
     return SPG_OK;
 }
@@ -252,16 +265,18 @@ struct Prof {
         }
         return c.events[c.events_used++];
     }
-    void begin(const char* name, int64_t rows, int grid) {
+    hipStream_t st = nullptr;
+    void begin(const std::string& name, int64_t rows, int grid, hipStream_t on = nullptr) {
         if (!c.profiling) return;
+        st = on ? on : c.stream;
         ProfRec r{name, rows, grid, next_event(), next_event(), 0.0};
-        if (r.ev0) (void)hipEventRecord(r.ev0, c.stream);
+        if (r.ev0) (void)hipEventRecord(r.ev0, st);
         c.prof.push_back(r);
         open = static_cast<int>(c.prof.size()) - 1;
     }
     void end() {
         if (!c.profiling || open < 0) return;
-        if (c.prof[open].ev1) (void)hipEventRecord(c.prof[open].ev1, c.stream);
+        if (c.prof[open].ev1) (void)hipEventRecord(c.prof[open].ev1, st);
         open = -1;
     }
 };
@@ -542,32 +557,36 @@ spg_status multiply_impl(Context& c, int count, const spg_csr* A, const spg_csr*
         }
     }
 
-    // fused pass
-    MultiArgs ma{};
-    int item_off = 0;
-    int big_bin = -1, big_off = 0, glob_bin = -1, glob_off = 0;
-    for (int b = 0; b < c.num.n; ++b) {
-        const int cnt = hm.num_count[b];
-        const int v = c.num.variant[b];
-        if (cnt > 0) {
-            if (v < kNumMultiVariants) {
-                const int k = ma.n++;
-                ma.bin[k] = b;
-                ma.variant[k] = v;
-                ma.rows[k] = kNumVariants[v].r;
-                ma.item_off[k] = item_off;
-                ma.count[k] = cnt;
-                ma.unit_start[k + 1] = ma.unit_start[k] + (cnt + ma.rows[k] - 1) / ma.rows[k];
-            } else if (v == kNumBigVariant) {
-                big_bin = b;
-                big_off = item_off;
-            } else {
-                glob_bin = b;
-                glob_off = item_off;
+    // fused pass: one launch per non-empty bin, heaviest bins first, spread
+    // over the context's streams so that small bins overlap
+    struct Launch {
+        int variant, off, count;
+    };
+    std::vector<Launch> launches;
+    int glob_bin = -1, glob_off = 0;
+    {
+        int item_off = 0;
+        for (int b = 0; b < c.num.n; ++b) {
+            const int cnt = hm.num_count[b];
+            const int v = c.num.variant[b];
+            if (cnt > 0) {
+                if (v == kNumGlobalVariant) {
+                    glob_bin = b;
+                    glob_off = item_off;
+                } else {
+                    launches.push_back(Launch{v, item_off, cnt});
+                }
             }
+            item_off += cnt;
         }
-        item_off += cnt;
     }
+    std::stable_sort(launches.begin(), launches.end(), [](const Launch& x, const Launch& y) {
+        return kNumVariants[x.variant].p * kNumVariants[x.variant].wg / kNumVariants[x.variant].r >
+               kNumVariants[y.variant].p * kNumVariants[y.variant].wg / kNumVariants[y.variant].r;
+    });
+    const int n_launch = static_cast<int>(launches.size()) + (glob_bin >= 0 ? 1 : 0);
+    const int n_streams = std::min(n_launch, static_cast<int>(Context::kStreams));
+    auto stream_of = [&](int k) { return k % Context::kStreams == 0 ? s : c.aux[k % Context::kStreams - 1]; };
     void* num_pool = nullptr;
     if (glob_bin >= 0) {
         st = dev_alloc(c, &num_pool, sizeof(int32_t) * hm.gnum_pool);
@@ -575,38 +594,43 @@ spg_status multiply_impl(Context& c, int count, const spg_csr* A, const spg_csr*
             cleanup();
             return st;
         }
-        prof.begin("symnum_global", hm.num_count[glob_bin], hm.num_count[glob_bin]);
+    }
+    hipError_t e = hipSuccess;
+    if (n_streams > 1) {  // C.col, C.val and the pool were allocated on the main stream
+        e = hipEventRecord(c.ev_ready, s);
+        for (int i = 1; i < n_streams && e == hipSuccess; ++i) e = hipStreamWaitEvent(c.aux[i - 1], c.ev_ready, 0);
+    }
+    int k = 0;
+    if (e == hipSuccess && glob_bin >= 0) {
+        const hipStream_t on = stream_of(k++);
+        prof.begin("symnum_global", hm.num_count[glob_bin], hm.num_count[glob_bin], on);
         launch_symnum_global<V>(ps, items2, gpool_off, glob_off, hm.num_count[glob_bin],
-                                static_cast<int*>(num_pool), meta, s);
+                                static_cast<int*>(num_pool), meta, on);
         prof.end();
     }
-    if (big_bin >= 0) {
-        const int cnt = hm.num_count[big_bin];
-        const int grid = std::min(cnt, c.cus * c.occ_num_big[di]);
-        prof.begin("symnum_big", cnt, grid);
-        launch_symnum_big<V>(ps, items2, big_off, cnt, meta, grid, s);
-        prof.end();
-    }
-    if (ma.n > 0) {
-        const int units = ma.unit_start[ma.n];
-        int grid = std::min(units, c.cus * c.occ_num_multi[di]);
+    for (const Launch& L : launches) {
+        if (e != hipSuccess) break;
+        const NumVariant& nv = kNumVariants[L.variant];
+        const int units = (L.count + nv.r - 1) / nv.r;
+        int grid = std::min(units, c.cus * c.occ_num[di][L.variant]);
         grid = (grid + kXcds - 1) / kXcds * kXcds;
-        int64_t rows = 0;
-        for (int k = 0; k < ma.n; ++k) rows += ma.count[k];
-        prof.begin("symnum_multi", rows, grid);
-        launch_symnum_multi<V>(ps, items2, ma, meta, grid, s);
+        const hipStream_t on = stream_of(k++);
+        prof.begin("symnum_v" + std::to_string(L.variant), L.count, grid, on);
+        e = launch_symnum_bin<V>(L.variant, ps, items2, L.off, L.count, meta, grid, on);
         prof.end();
     }
-    {
-        hipError_t e = hipGetLastError();
-        if (e == hipSuccess) e = hipMemcpyAsync(&c.h_meta->flags, &meta->flags, sizeof(int32_t),
-                                                hipMemcpyDeviceToHost, s);
-        if (num_pool) (void)dev_release(c, num_pool);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess) {
-            cleanup();
-            return hip_fail(e, "fused pass", __FILE__, __LINE__);
-        }
+    for (int i = 1; i < n_streams && e == hipSuccess; ++i) {
+        e = hipEventRecord(c.ev_done[i - 1], c.aux[i - 1]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s, c.ev_done[i - 1], 0);
+    }
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&c.h_meta->flags, &meta->flags, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (num_pool) (void)dev_release(c, num_pool);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        cleanup();
+        return hip_fail(e, "fused pass", __FILE__, __LINE__);
     }
     if (c.h_meta->flags & kFlagMismatch) {
         cleanup();
@@ -797,13 +821,13 @@ spg_status spg_get_variants(int32_t phase, char* buf, size_t buf_len) {
     std::ostringstream out;
     if (phase == 0) {
         for (int v = 0; v <= kSymMaxVariant; ++v)
-            out << v << " wg=" << kSymVariants[v].wg << " rows=" << kSymVariants[v].r << " lanes_per_entry="
-                << kSymVariants[v].g << " table=" << kSymVariants[v].t << (v == kSymMaxVariant ? " max" : "")
-                << '\n';
+            out << v << " wg=" << kSymVariants[v].wg << " rows=" << kSymVariants[v].r << " table=" << kSymVariants[v].t
+                << " chunk=" << kSymVariants[v].ec << " slots=" << kSymVariants[v].pmax
+                << (v == kSymMaxVariant ? " max" : "") << '\n';
     } else if (phase == 1) {
         for (int v = 0; v < kNumGlobalVariant; ++v)
-            out << v << " wg=" << kNumVariants[v].wg << " rows=" << kNumVariants[v].r << " lanes_per_entry="
-                << kNumVariants[v].g << " capacity=" << kNumVariants[v].p << '\n';
+            out << v << " wg=" << kNumVariants[v].wg << " rows=" << kNumVariants[v].r << " capacity="
+                << kNumVariants[v].p << " chunk=" << kNumVariants[v].ec << " slots=" << kNumVariants[v].pmax << '\n';
         out << kNumGlobalVariant << " global\n";
     } else {
         return fail(SPG_ERR_INVALID_ARGUMENT, "phase must be 0 or 1");

@@ -1,62 +1,57 @@
 // Symbolic count: exact nnz of every row of C from an open-addressing hash
 // table of the row's B columns (sym1.cuh:28-306), written into c_rpt.
 //
-// Work decomposition: a workgroup unit holds R rows; each row owns L = WG/R
-// consecutive lanes; those lanes split into L/G sub-groups that take the
-// row's A entries in turn, and the G lanes of a sub-group stride the B row
-// of that entry.  All tables live in LDS except for rows that overflow the
-// largest table, which are recounted in global memory.
+// A workgroup unit holds R rows; each row owns L = WG/R consecutive lanes
+// that enumerate its products flat (spg_rows.hpp) and insert their columns
+// into the row's LDS table.  Rows that overflow the largest table are
+// recounted in global memory.
 #include "spg_kernels.hpp"
+#include "spg_rows.hpp"
 
 namespace spg {
 
-template <typename V, int WG, int R, int G, int T>
+template <typename V, int WG, int R, int T, int EC, int PMAX>
 __device__ __forceinline__ void sym_rows(const ProbSet<V>& ps, const int2* __restrict__ items, int first,
-                                         int end, int* smem, int* s_red) {
+                                         int end, int* smem, int* s_tmp) {
     constexpr int L = WG / R;
-    constexpr int E = L / G;
-    static_assert(WG % R == 0 && L % G == 0, "bad variant shape");
+    static_assert(WG % R == 0, "bad variant shape");
     static_assert(L <= kWave || R == 1, "a row spans several waves only when it owns the workgroup");
+    using Meta = ChunkMeta<V, EC, false>;
     const int slot = threadIdx.x / L;
     const int lr = threadIdx.x % L;
-    const int sub = lr / G;
-    const int g = lr % G;
     int* tab = smem + slot * T;
-    for (int j = lr; j < T; j += L) tab[j] = kEmpty;
-    if constexpr (L > kWave) {
-        if (threadIdx.x == 0) *s_red = 0;
-    }
-    __syncthreads();
+    Meta* cm = reinterpret_cast<Meta*>(smem + R * T) + slot;
+    lds_fill<T, L>(tab, kEmpty, lr);
+    gsync<L>();
 
     const int idx = first + slot;
     const bool active = idx < end;
     const int2 it = active ? items[idx] : make_int2(0, 0);
     const Prob<V> P = ps.get(it.x);
-    int mine = 0;
+    int a0 = 0, a1 = 0;
     if (active) {
-        const int a1 = P.a_rpt[it.y + 1];
-        for (int i = P.a_rpt[it.y] + sub; i < a1; i += E) {
-            const int k = P.a_col[i];
-            const int q1 = P.b_rpt[k + 1];
-            for (int q = P.b_rpt[k] + g; q < q1; q += G) mine += table_insert<T>(tab, P.b_col[q]);
-        }
+        a0 = P.a_rpt[it.y];
+        a1 = P.a_rpt[it.y + 1];
     }
+    int mine = 0;
+    for_each_product<V, L, EC, PMAX, false>(P, a0, a1, lr, cm, s_tmp,
+                                            [&](int key, V) { mine += table_insert<T>(tab, key); });
     int nnz;
     if constexpr (L <= kWave) {
         nnz = group_sum<L>(mine);
     } else {
-        if (mine) atomicAdd(s_red, mine);
-        __syncthreads();
-        nnz = *s_red;
+        int total;
+        (void)group_excl_scan<L>(mine, lr, s_tmp, total);
+        nnz = total;
     }
     if (active && lr == 0) P.c_rpt[it.y] = nnz;
-    __syncthreads();  // tables are reused by the next unit
+    gsync<L>();  // the table is reused by the next unit
 }
 
-#define SPG_SYM_CASE(v)                                                                                     \
-    case v:                                                                                                 \
-        sym_rows<V, kMultiThreads, kSymVariants[v].r, kSymVariants[v].g, kSymVariants[v].t>(ps, items, first, \
-                                                                                            end, smem, &s_red); \
+#define SPG_SYM_CASE(v)                                                                                   \
+    case v:                                                                                               \
+        sym_rows<V, kMultiThreads, kSymVariants[v].r, kSymVariants[v].t, kSymVariants[v].ec,              \
+                 kSymVariants[v].pmax>(ps, items, first, end, smem, s_tmp);                               \
         break;
 
 // One launch serves bins [0, n_multi) of the symbolic table.  Bin sizes are
@@ -67,7 +62,7 @@ __global__ __launch_bounds__(kMultiThreads) void k_sym_multi(ProbSet<V> ps, cons
                                                              BinTable sym, int n_multi,
                                                              const DevMeta* __restrict__ meta) {
     __shared__ __align__(16) int smem[kSymMultiLds / 4];
-    __shared__ int s_red;
+    __shared__ int s_tmp[kMultiThreads / kWave];
     __shared__ int s_unit[kMaxBins + 1];
     __shared__ int s_off[kMaxBins];
     __shared__ int s_cnt[kMaxBins];
@@ -99,9 +94,14 @@ __global__ __launch_bounds__(kMultiThreads) void k_sym_multi(ProbSet<V> ps, cons
             SPG_SYM_CASE(5)
             SPG_SYM_CASE(6)
             SPG_SYM_CASE(7)
+            SPG_SYM_CASE(8)
             default:
                 break;
         }
+        // Consecutive units may belong to bins with different LDS layouts,
+        // and rows of L <= 64 lanes only synchronise inside their wave: no
+        // wave may start the next unit while another still uses this one.
+        __syncthreads();
     }
 }
 
@@ -113,44 +113,46 @@ template <typename V>
 __global__ __launch_bounds__(1024) void k_sym_max(ProbSet<V> ps, const int2* __restrict__ items, int bin,
                                                   DevMeta* __restrict__ meta, int2* __restrict__ fail_items,
                                                   long long* __restrict__ fail_off) {
-    constexpr int T = kSymVariants[kSymMaxVariant].t;
-    constexpr int G = kSymVariants[kSymMaxVariant].g;
+    constexpr SymVariant kV = kSymVariants[kSymMaxVariant];
+    constexpr int T = kV.t;
     constexpr int WG = 1024;
-    constexpr int E = WG / G;
-    static_assert(T - kSymMaxThreshold > WG + 1, "table must keep room for in-flight inserts");
-    __shared__ int tab[T];
+    constexpr int EC = kV.ec;
+    constexpr int PMAX = kV.pmax;
+    static_assert(T - kSymMaxThreshold > WG * PMAX + 1, "table must keep room for in-flight inserts");
+    using Meta = ChunkMeta<V, EC, false>;
+    __shared__ __align__(16) int tab[T];
+    __shared__ Meta cm;
+    __shared__ int s_tmp[WG / kWave];
     __shared__ int s_n;
-    __shared__ int s_abort;
     int off = 0;
     for (int b = 0; b < bin; ++b) off += meta->sym_count[b];
     const int count = meta->sym_count[bin];
-    const int sub = threadIdx.x / G;
-    const int g = threadIdx.x % G;
     for (int u = blockIdx.x; u < count; u += gridDim.x) {
-        for (int j = threadIdx.x; j < T; j += WG) tab[j] = kEmpty;
-        if (threadIdx.x == 0) {
-            s_n = 0;
-            s_abort = 0;
-        }
+        lds_fill<T, WG>(tab, kEmpty, threadIdx.x);
+        if (threadIdx.x == 0) s_n = 0;
         __syncthreads();
         const int2 it = items[off + u];
         const Prob<V> P = ps.get(it.x);
+        const int a0 = P.a_rpt[it.y];
         const int a1 = P.a_rpt[it.y + 1];
-        for (int i = P.a_rpt[it.y] + sub; i < a1; i += E) {
-            const int k = P.a_col[i];
-            const int q1 = P.b_rpt[k + 1];
-            bool stop = false;
-            for (int q = P.b_rpt[k] + g; q < q1; q += G) {
-                if (lds_load(&s_abort)) {
-                    stop = true;
-                    break;
-                }
-                if (table_insert<T>(tab, P.b_col[q]) && atomicAdd(&s_n, 1) >= kSymMaxThreshold)
-                    __hip_atomic_store(&s_abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        bool over = false;
+        for (int ebase = a0; ebase < a1 && !over; ebase += EC) {
+            const int total = load_chunk<V, WG, EC, false>(P, ebase, a1, threadIdx.x, &cm, s_tmp);
+            for (int base = 0; base < total && !over; base += WG * PMAX) {
+                int key[PMAX];
+                V prod[PMAX];
+                bool ok[PMAX];
+                fetch_products<V, WG, EC, PMAX, false>(P, &cm, total, base, threadIdx.x, key, prod, ok);
+                int mine = 0;
+#pragma unroll
+                for (int j = 0; j < PMAX; ++j)
+                    if (ok[j]) mine += table_insert<T>(tab, key[j]);
+                if (mine) atomicAdd(&s_n, mine);
+                __syncthreads();
+                over = s_n > kSymMaxThreshold;  // uniform: every lane reads after the barrier
+                __syncthreads();
             }
-            if (stop) break;
         }
-        __syncthreads();
         if (threadIdx.x == 0) {
             const int n = s_n;
             if (n <= kSymMaxThreshold) {
@@ -173,47 +175,42 @@ template <typename V>
 __global__ __launch_bounds__(1024) void k_sym_global(ProbSet<V> ps, const int2* __restrict__ fail_items,
                                                      const long long* __restrict__ fail_off, int* pool) {
     constexpr int WG = 1024;
-    constexpr int G = 64;
-    constexpr int E = WG / G;
-    __shared__ int s_n;
+    constexpr int EC = 1024;
+    constexpr int PMAX = 4;
+    using Meta = ChunkMeta<V, EC, false>;
+    __shared__ Meta cm;
+    __shared__ int s_tmp[WG / kWave];
     const int2 it = fail_items[blockIdx.x];
     const Prob<V> P = ps.get(it.x);
     const long long slots = global_table_slots(P.nip[it.y]);
     const int bits = __builtin_ctzll(static_cast<unsigned long long>(slots));
     int* tab = pool + fail_off[blockIdx.x];
     for (long long j = threadIdx.x; j < slots; j += WG) tab[j] = kEmpty;
-    if (threadIdx.x == 0) s_n = 0;
     __threadfence();
     __syncthreads();
-    const int sub = threadIdx.x / G;
-    const int g = threadIdx.x % G;
     const uint32_t mask = static_cast<uint32_t>(slots - 1);
     int mine = 0;
-    const int a1 = P.a_rpt[it.y + 1];
-    for (int i = P.a_rpt[it.y] + sub; i < a1; i += E) {
-        const int k = P.a_col[i];
-        const int q1 = P.b_rpt[k + 1];
-        for (int q = P.b_rpt[k] + g; q < q1; q += G) {
-            const int key = P.b_col[q];
-            uint32_t h = hash_slot_rt(key, bits);
-            while (true) {
-                const int cur = __hip_atomic_load(tab + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (cur == key) break;
-                if (cur == kEmpty) {
-                    const int old = atomicCAS(tab + h, kEmpty, key);
-                    if (old == kEmpty) {
-                        ++mine;
-                        break;
-                    }
-                    if (old == key) break;
-                }
-                h = (h + 1) & mask;
-            }
-        }
-    }
-    if (mine) atomicAdd(&s_n, mine);
-    __syncthreads();
-    if (threadIdx.x == 0) P.c_rpt[it.y] = s_n;
+    for_each_product<V, WG, EC, PMAX, false>(P, P.a_rpt[it.y], P.a_rpt[it.y + 1], threadIdx.x, &cm, s_tmp,
+                                             [&](int key, V) {
+                                                 uint32_t h = hash_slot_rt(key, bits);
+                                                 while (true) {
+                                                     const int cur = __hip_atomic_load(
+                                                         tab + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                                     if (cur == key) break;
+                                                     if (cur == kEmpty) {
+                                                         const int old = atomicCAS(tab + h, kEmpty, key);
+                                                         if (old == kEmpty) {
+                                                             ++mine;
+                                                             break;
+                                                         }
+                                                         if (old == key) break;
+                                                     }
+                                                     h = (h + 1) & mask;
+                                                 }
+                                             });
+    int total;
+    (void)group_excl_scan<WG>(mine, threadIdx.x, s_tmp, total);
+    if (threadIdx.x == 0) P.c_rpt[it.y] = total;
 }
 
 // ----------------------------------------------------------------- launchers

@@ -10,151 +10,150 @@
 //      num.cuh:28-41) and add the product there (num.cuh:104-116);
 //   4. write the values to C.val.
 // Fusing the two passes keeps the sorted column list in LDS between them,
-// so C.col is written once and never read back.
+// so C.col is written once and never read back.  A row whose products fit
+// in one chunk and in the lanes' PMAX register slots keeps (column, a*b)
+// in registers from step 1 to step 3 and reads A and B only once.
 #include "spg_kernels.hpp"
+#include "spg_rows.hpp"
 
 namespace spg {
 
-// Ascending bitonic sort of P (a power of two) ints in LDS by L lanes.
-template <int P, int L>
-__device__ __forceinline__ void bitonic_sort(int* cols, int lr) {
-    for (int k = 2; k <= P; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int t = lr; t < P / 2; t += L) {
-                const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-                const int a = cols[i];
-                const int b = cols[i + j];
-                const bool asc = (i & k) == 0;
-                if ((a > b) == asc) {
-                    cols[i] = b;
-                    cols[i + j] = a;
-                }
+// Compact the occupied slots of a row's table into cols[0..n) (unsorted);
+// returns the number of occupied slots.
+template <int L, int T, int P>
+__device__ __forceinline__ int compact_table(const int* tab, int* cols, int lr, int* s_tmp) {
+    static_assert(T % L == 0, "table must be a multiple of the group");
+    int n = 0;
+    for (int j0 = 0; j0 < T; j0 += L) {
+        const int key = tab[j0 + lr];
+        const bool has = key != kEmpty;
+        if constexpr (L == 1) {
+            if (has) {
+                if (n < P) cols[n] = key;
+                ++n;
             }
-            __syncthreads();
+        } else if constexpr (L <= kWave) {
+            const unsigned long long m = __ballot(has);
+            const int shift = (threadIdx.x % kWave) / L * L;
+            const unsigned long long seg = L == kWave ? m : (m >> shift) & ((1ull << L) - 1);
+            const int pos = n + __popcll(seg & ((1ull << lr) - 1));
+            if (has && pos < P) cols[pos] = key;
+            n += __popcll(seg);
+        } else {
+            int tot;
+            const int ex = group_excl_scan<L>(has ? 1 : 0, lr, s_tmp, tot);
+            if (has && n + ex < P) cols[n + ex] = key;
+            n += tot;
         }
     }
+    return n;
 }
 
-template <typename V, int WG, int R, int G, int P>
+template <typename V, int WG, int R, int P, int EC, int PMAX>
 __device__ __forceinline__ void symnum_rows(const ProbSet<V>& ps, const int2* __restrict__ items, int first,
-                                            int end, int* smem, int* s_cnt, DevMeta* meta) {
+                                            int end, int* smem, int* s_tmp, DevMeta* meta) {
     constexpr int L = WG / R;
-    constexpr int E = L / G;
     constexpr int T = 2 * P;
-    static_assert(WG % R == 0 && L % G == 0, "bad variant shape");
+    constexpr int ROW_INTS = num_row_lds(NumVariant{WG, R, P, EC, PMAX}) / 4;
+    static_assert(WG % R == 0, "bad variant shape");
+    static_assert(L <= kWave || R == 1, "a row spans several waves only when it owns the workgroup");
     static_assert(T * sizeof(int) >= P * sizeof(V), "values must fit in the table's LDS");
+    static_assert(ROW_INTS % 4 == 0, "row slots must stay 16-byte aligned");
+    using Meta = ChunkMeta<V, EC, true>;
+    using KeyMeta = ChunkMeta<V, EC, false>;
     const int slot = threadIdx.x / L;
     const int lr = threadIdx.x % L;
-    const int sub = lr / G;
-    const int g = lr % G;
-    int* tab = smem + slot * T;
-    int* cols = smem + R * T + slot * P;
+    int* tab = smem + slot * ROW_INTS;
+    int* cols = tab + T;
+    Meta* cm = reinterpret_cast<Meta*>(cols + P);
     V* vals = reinterpret_cast<V*>(tab);
 
-    for (int j = lr; j < T; j += L) tab[j] = kEmpty;
-    if (lr == 0) s_cnt[slot] = 0;
-    __syncthreads();
+    lds_fill<T, L>(tab, kEmpty, lr);
+    gsync<L>();
 
     const int idx = first + slot;
     const bool active = idx < end;
     const int2 it = active ? items[idx] : make_int2(0, 0);
     const Prob<V> Pr = ps.get(it.x);
-    int a0 = 0, a1 = 0;
+    int a0 = 0, a1 = 0, nip = 0;
     if (active) {
         a0 = Pr.a_rpt[it.y];
         a1 = Pr.a_rpt[it.y + 1];
+        nip = Pr.nip[it.y];
     }
-    for (int i = a0 + sub; i < a1; i += E) {
-        const int k = Pr.a_col[i];
-        const int q1 = Pr.b_rpt[k + 1];
-        for (int q = Pr.b_rpt[k] + g; q < q1; q += G) table_insert<T>(tab, Pr.b_col[q]);
-    }
-    __syncthreads();
-
-    for (int j = lr; j < T; j += L) {
-        const int key = tab[j];
-        if (key != kEmpty) {
-            const int pos = atomicAdd(&s_cnt[slot], 1);
-            if (pos < P) cols[pos] = key;
+    // products staged in registers when the whole row fits in one round
+    const bool fits = a1 - a0 <= EC && nip <= L * PMAX;
+    int key[PMAX];
+    V prod[PMAX];
+    bool ok[PMAX];
+    if (fits) {
+        int total;
+        if constexpr (L == 1) {
+            Meta rm;
+            total = load_chunk_lane<V, EC, true>(Pr, a0, a1, rm);
+            fetch_products_lane<V, EC, PMAX, true>(Pr, rm, total, 0, key, prod, ok);
+        } else {
+            total = load_chunk<V, L, EC, true>(Pr, a0, a1, lr, cm, s_tmp);
+            fetch_products<V, L, EC, PMAX, true>(Pr, cm, total, 0, lr, key, prod, ok);
         }
+#pragma unroll
+        for (int j = 0; j < PMAX; ++j)
+            if (ok[j]) table_insert<T>(tab, key[j]);
+    } else {
+        for_each_product<V, L, EC, PMAX, false>(Pr, a0, a1, lr, reinterpret_cast<KeyMeta*>(cm), s_tmp,
+                                                [&](int k, V) { table_insert<T>(tab, k); });
     }
-    __syncthreads();
-    const int n = min(s_cnt[slot], P);
+    gsync<L>();
+
+    const int found = compact_table<L, T, P>(tab, cols, lr, s_tmp);
+    const int n = min(found, P);
     for (int j = n + lr; j < P; j += L) cols[j] = INT_MAX;
-    __syncthreads();
+    gsync<L>();
     bitonic_sort<P, L>(cols, lr);
 
     int c0 = 0, w = 0;
     if (active) {
         c0 = Pr.c_rpt[it.y];
         const int cn = Pr.c_rpt[it.y + 1] - c0;
-        if (lr == 0 && cn != s_cnt[slot]) atomicOr(&meta->flags, kFlagMismatch);
+        if (lr == 0 && cn != found) atomicOr(&meta->flags, kFlagMismatch);
         w = min(n, cn);
     }
     for (int j = lr; j < w; j += L) Pr.c_col[c0 + j] = cols[j];
     for (int j = lr; j < P; j += L) vals[j] = V(0);
-    __syncthreads();
+    gsync<L>();
 
-    for (int i = a0 + sub; i < a1; i += E) {
-        const int k = Pr.a_col[i];
-        const V av = Pr.a_val[i];
-        const int q1 = Pr.b_rpt[k + 1];
-        for (int q = Pr.b_rpt[k] + g; q < q1; q += G) {
-            const int pos = lower_bound_pow2<P>(cols, Pr.b_col[q]);
-            const V prod = av * Pr.b_val[q];
-            if constexpr (L == 1)
-                vals[pos] += prod;
-            else
-                lds_add(vals + pos, prod);
-        }
+    auto accumulate = [&](int k, V v) {
+        const int pos = lower_bound_pow2<P>(cols, k);
+        if constexpr (L == 1)
+            vals[pos] += v;
+        else
+            lds_add(vals + pos, v);
+    };
+    if (fits) {
+#pragma unroll
+        for (int j = 0; j < PMAX; ++j)
+            if (ok[j]) accumulate(key[j], prod[j]);
+    } else {
+        for_each_product<V, L, EC, PMAX, true>(Pr, a0, a1, lr, cm, s_tmp, accumulate);
     }
-    __syncthreads();
+    gsync<L>();
     for (int j = lr; j < w; j += L) Pr.c_val[c0 + j] = vals[j];
-    __syncthreads();  // LDS is reused by the next unit
+    gsync<L>();  // LDS is reused by the next unit
 }
 
-#define SPG_NUM_CASE(v)                                                                                   \
-    case v:                                                                                               \
-        symnum_rows<V, kMultiThreads, kNumVariants[v].r, kNumVariants[v].g, kNumVariants[v].p>(            \
-            ps, items, first, end, smem, s_cnt, meta);                                                    \
-        break;
-
-template <typename V>
-__global__ __launch_bounds__(kMultiThreads) void k_symnum_multi(ProbSet<V> ps, const int2* __restrict__ items,
-                                                                MultiArgs ma, DevMeta* meta) {
-    __shared__ __align__(16) int smem[kNumMultiLds / 4];
-    __shared__ int s_cnt[kMultiThreads];
-    const UnitRange ur = xcd_units(ma.unit_start[ma.n]);
-    for (int u = ur.first; u < ur.end; u += ur.stride) {
-        int b = 0;
-        while (u >= ma.unit_start[b + 1]) ++b;
-        const int first = ma.item_off[b] + (u - ma.unit_start[b]) * ma.rows[b];
-        const int end = ma.item_off[b] + ma.count[b];
-        switch (ma.variant[b]) {
-            SPG_NUM_CASE(0)
-            SPG_NUM_CASE(1)
-            SPG_NUM_CASE(2)
-            SPG_NUM_CASE(3)
-            SPG_NUM_CASE(4)
-            SPG_NUM_CASE(5)
-            SPG_NUM_CASE(6)
-            SPG_NUM_CASE(7)
-            SPG_NUM_CASE(8)
-            default:
-                break;
-        }
-    }
-}
-
-// One 1024-lane workgroup per row, P = 8192 (96 KiB of LDS in FP64).
-template <typename V>
-__global__ __launch_bounds__(1024) void k_symnum_big(ProbSet<V> ps, const int2* __restrict__ items, int item_off,
-                                                     int count, DevMeta* meta) {
-    constexpr NumVariant kV = kNumVariants[kNumBigVariant];
-    __shared__ __align__(16) int smem[kNumBigLds / 4];
-    __shared__ int s_cnt[1];
-    for (int u = blockIdx.x; u < count; u += gridDim.x)
-        symnum_rows<V, 1024, kV.r, kV.g, kV.p>(ps, items, item_off + u, item_off + count, smem, s_cnt, meta);
+// One launch per fused variant: a persistent grid walks the units (R rows
+// each) of one bin; consecutive units stay on one XCD.
+template <typename V, int VV>
+__global__ __launch_bounds__(kNumVariants[VV].wg) void k_symnum_bin(ProbSet<V> ps, const int2* __restrict__ items,
+                                                                    int item_off, int count, DevMeta* meta) {
+    constexpr NumVariant kV = kNumVariants[VV];
+    __shared__ __align__(16) int smem[kV.r * num_row_lds(kV) / 4];
+    __shared__ int s_tmp[kV.wg / kWave];
+    const int units = (count + kV.r - 1) / kV.r;
+    const UnitRange ur = xcd_units(units);
+    for (int u = ur.first; u < ur.end; u += ur.stride)
+        symnum_rows<V, kV.wg, kV.r, kV.p, kV.ec, kV.pmax>(ps, items, item_off + u * kV.r, item_off + count, smem,
+                                                          s_tmp, meta);
 }
 
 // Lower bound in a sorted global list (values read coherently).
@@ -181,10 +180,13 @@ __global__ __launch_bounds__(1024) void k_symnum_global(ProbSet<V> ps, const int
                                                         const long long* __restrict__ gpool_off, int item_off,
                                                         int* pool, DevMeta* meta) {
     constexpr int WG = 1024;
-    constexpr int G = 64;
-    constexpr int E = WG / G;
+    constexpr int EC = 1024;
+    constexpr int PMAX = 4;
     constexpr int CH = kGlobalSortChunk;
-    __shared__ int chunk[CH];
+    using Meta = ChunkMeta<V, EC, true>;
+    __shared__ __align__(16) int chunk[CH];
+    __shared__ Meta cm;
+    __shared__ int s_tmp[WG / kWave];
     __shared__ int s_n;
     const int2 it = items[item_off + blockIdx.x];
     const Prob<V> P = ps.get(it.x);
@@ -202,15 +204,10 @@ __global__ __launch_bounds__(1024) void k_symnum_global(ProbSet<V> ps, const int
     __threadfence();
     __syncthreads();
 
-    const int sub = threadIdx.x / G;
-    const int g = threadIdx.x % G;
     const int a0 = P.a_rpt[it.y];
     const int a1 = P.a_rpt[it.y + 1];
-    for (int i = a0 + sub; i < a1; i += E) {
-        const int k = P.a_col[i];
-        const int q1 = P.b_rpt[k + 1];
-        for (int q = P.b_rpt[k] + g; q < q1; q += G) {
-            const int key = P.b_col[q];
+    for_each_product<V, WG, EC, PMAX, false>(
+        P, a0, a1, threadIdx.x, reinterpret_cast<ChunkMeta<V, EC, false>*>(&cm), s_tmp, [&](int key, V) {
             uint32_t h = hash_slot_rt(key, bits);
             while (true) {
                 const int cur = __hip_atomic_load(tab + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -221,8 +218,7 @@ __global__ __launch_bounds__(1024) void k_symnum_global(ProbSet<V> ps, const int
                 }
                 h = (h + 1) & mask;
             }
-        }
-    }
+        });
     __threadfence();
     __syncthreads();
 
@@ -268,30 +264,44 @@ __global__ __launch_bounds__(1024) void k_symnum_global(ProbSet<V> ps, const int
     __threadfence();
     __syncthreads();
 
-    for (int i = a0 + sub; i < a1; i += E) {
-        const int k = P.a_col[i];
-        const V av = P.a_val[i];
-        const int q1 = P.b_rpt[k + 1];
-        for (int q = P.b_rpt[k] + g; q < q1; q += G) {
-            const int pos = lower_bound_global(ccol, n, P.b_col[q]);
-            if (pos < n)
-                __hip_atomic_fetch_add(cval + pos, av * P.b_val[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
+    for_each_product<V, WG, EC, PMAX, true>(P, a0, a1, threadIdx.x, &cm, s_tmp, [&](int key, V v) {
+        const int pos = lower_bound_global(ccol, n, key);
+        if (pos < n) __hip_atomic_fetch_add(cval + pos, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    });
 }
 
 // ----------------------------------------------------------------- launchers
 
-template <typename V>
-void launch_symnum_multi(const ProbSet<V>& ps, const int2* items, const MultiArgs& ma, DevMeta* meta, int grid,
-                         hipStream_t s) {
-    hipLaunchKernelGGL(k_symnum_multi<V>, dim3(grid), dim3(kMultiThreads), 0, s, ps, items, ma, meta);
+template <typename V, int VV>
+static hipError_t launch_variant(const ProbSet<V>& ps, const int2* items, int item_off, int count, DevMeta* meta,
+                                 int grid, hipStream_t s) {
+    constexpr auto kernel = k_symnum_bin<V, VV>;
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kNumVariants[VV].wg), 0, s, ps, items, item_off, count, meta);
+    return hipGetLastError();
 }
 
+template <typename V, int VV>
+static int occupancy_variant() {
+    constexpr auto kernel = k_symnum_bin<V, VV>;
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, kNumVariants[VV].wg, 0) != hipSuccess) n = 1;
+    return n > 0 ? n : 1;
+}
+
+#define SPG_NUM_VARIANTS(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9)
+
 template <typename V>
-void launch_symnum_big(const ProbSet<V>& ps, const int2* items, int item_off, int count, DevMeta* meta,
-                       int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_symnum_big<V>, dim3(grid), dim3(1024), 0, s, ps, items, item_off, count, meta);
+hipError_t launch_symnum_bin(int variant, const ProbSet<V>& ps, const int2* items, int item_off, int count,
+                             DevMeta* meta, int grid, hipStream_t s) {
+#define SPG_LAUNCH_CASE(v) \
+    case v:                \
+        return launch_variant<V, v>(ps, items, item_off, count, meta, grid, s);
+    switch (variant) {
+        SPG_NUM_VARIANTS(SPG_LAUNCH_CASE)
+        default:
+            return hipErrorInvalidValue;
+    }
+#undef SPG_LAUNCH_CASE
 }
 
 template <typename V>
@@ -302,28 +312,24 @@ void launch_symnum_global(const ProbSet<V>& ps, const int2* items, const long lo
 }
 
 template <typename V>
-int occupancy_symnum_multi() {
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_symnum_multi<V>, kMultiThreads, 0) != hipSuccess)
-        n = 1;
-    return n > 0 ? n : 1;
+int occupancy_symnum_bin(int variant) {
+#define SPG_OCC_CASE(v) \
+    case v:             \
+        return occupancy_variant<V, v>();
+    switch (variant) {
+        SPG_NUM_VARIANTS(SPG_OCC_CASE)
+        default:
+            return 1;
+    }
+#undef SPG_OCC_CASE
 }
 
-template <typename V>
-int occupancy_symnum_big() {
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_symnum_big<V>, 1024, 0) != hipSuccess) n = 1;
-    return n > 0 ? n : 1;
-}
-
-#define SPG_INSTANTIATE_NUM(V)                                                                              \
-    template void launch_symnum_multi<V>(const ProbSet<V>&, const int2*, const MultiArgs&, DevMeta*, int,    \
-                                         hipStream_t);                                                       \
-    template void launch_symnum_big<V>(const ProbSet<V>&, const int2*, int, int, DevMeta*, int, hipStream_t); \
-    template void launch_symnum_global<V>(const ProbSet<V>&, const int2*, const long long*, int, int, int*,   \
-                                          DevMeta*, hipStream_t);                                            \
-    template int occupancy_symnum_multi<V>();                                                                \
-    template int occupancy_symnum_big<V>();
+#define SPG_INSTANTIATE_NUM(V)                                                                               \
+    template hipError_t launch_symnum_bin<V>(int, const ProbSet<V>&, const int2*, int, int, DevMeta*, int,       \
+                                             hipStream_t);                                                    \
+    template void launch_symnum_global<V>(const ProbSet<V>&, const int2*, const long long*, int, int, int*,    \
+                                          DevMeta*, hipStream_t);                                             \
+    template int occupancy_symnum_bin<V>(int);
 
 SPG_INSTANTIATE_NUM(float)
 SPG_INSTANTIATE_NUM(double)

@@ -13,40 +13,51 @@
 namespace spg {
 
 struct SymVariant {
-    int wg, r, g, t;  // workgroup, rows per workgroup, lanes per A entry, table slots
+    int wg, r, t, ec, pmax;  // workgroup, rows per workgroup, table slots, A entries per chunk, products per lane
 };
 struct NumVariant {
-    int wg, r, g, p;  // ... p = sorted-list capacity (table has 2p slots)
+    int wg, r, p, ec, pmax;  // ... p = sorted-list capacity (the hash table has 2p slots)
 };
 
-// Symbolic (count) variants.  0..7 run inside one multi-bin launch of 256
-// threads; 8 is the largest LDS table with overflow detection (sym1.cuh:169-245).
+// Symbolic (count) variants.  0..8 run inside one multi-bin launch of 256
+// threads; 9 is the largest LDS table with overflow detection
+// (sym1.cuh:169-245).  A bin may take rows of NIP below the table size.
 constexpr SymVariant kSymVariants[] = {
-    {256, 256, 1, 16},  {256, 128, 1, 32}, {256, 64, 1, 64},  {256, 32, 2, 128},
-    {256, 16, 4, 256},  {256, 4, 8, 512},  {256, 4, 16, 2048}, {256, 1, 32, 8192},
-    {1024, 1, 64, 32768},
+    {256, 256, 16, 4, 8},  {256, 128, 32, 4, 8},  {256, 64, 64, 8, 8},    {256, 32, 128, 16, 8},
+    {256, 16, 256, 32, 8}, {256, 4, 512, 64, 8},  {256, 4, 1024, 64, 12}, {256, 1, 4096, 256, 8},
+    {256, 1, 8192, 256, 8}, {1024, 1, 32768, 1024, 4},
 };
-constexpr int kSymMultiVariants = 8;
-constexpr int kSymMaxVariant = 8;
-constexpr int kSymGlobalVariant = 9;     // rows that overflowed the largest table
+constexpr int kSymMultiVariants = 9;
+constexpr int kSymMaxVariant = 9;
 constexpr int kSymMaxThreshold = 26214;  // distinct keys admitted in the 32768-slot table (80 %)
-constexpr int kSymMultiLds = 32768;      // bytes: max over variants 0..7 of r * t * 4
+constexpr int kSymMultiLds = 36 * 1024;  // bytes: max over variants 0..8 of r * (4t + 8ec + 4)
 
-// Fused sorted-columns + numeric variants.  0..8 share one multi-bin launch
-// of 256 threads; 9 is a 1024-thread workgroup per row; rows above its
-// capacity use global-memory tables.
+// Fused sorted-columns + numeric variants, one kernel each: 0..8 use 256
+// threads, 9 a 1024-thread workgroup per row; rows above its capacity use
+// global-memory tables (variant 10).  A row whose products fit in one
+// chunk and in lanes * pmax keeps them in registers between the two halves.
 constexpr NumVariant kNumVariants[] = {
-    {256, 256, 1, 8},   {256, 128, 1, 16}, {256, 64, 1, 32},  {256, 32, 2, 64},
-    {256, 16, 4, 128},  {256, 4, 16, 128}, {256, 4, 16, 512}, {256, 1, 32, 1024},
-    {256, 1, 64, 2048}, {1024, 1, 64, 8192},
+    {256, 256, 8, 4, 16},   {256, 128, 16, 4, 16}, {256, 64, 32, 8, 16},  {256, 32, 64, 16, 16},
+    {256, 16, 128, 32, 16}, {256, 4, 128, 64, 12}, {256, 4, 512, 64, 16}, {256, 1, 1024, 256, 8},
+    {256, 1, 2048, 256, 8}, {1024, 1, 8192, 1024, 8},
 };
-constexpr int kNumMultiVariants = 9;
-constexpr int kNumBigVariant = 9;
+constexpr int kNumLdsVariants = 10;  // variants with an LDS kernel (0..9)
 constexpr int kNumGlobalVariant = 10;
-constexpr int kNumMultiLds = 24576;  // bytes: max over variants 0..8 of r * 12 * p
-constexpr int kNumBigLds = 12 * 8192;
 
 constexpr int kGlobalSortChunk = 4096;  // LDS chunk of the global-bin column sort
+
+// LDS bytes of one row slot of a fused variant (FP64 sizing): table 2p
+// ints (reused for p values), sorted list p ints, chunk metadata.
+constexpr int num_row_lds(const NumVariant& v) { return 12 * v.p + (v.wg / v.r > 1 ? 16 * v.ec : 0); }
+constexpr int sym_row_lds(const SymVariant& v) { return 4 * v.t + (v.wg / v.r > 1 ? 8 * v.ec + 8 : 0); }
+constexpr int max_sym_multi_lds() {
+    int m = 0;
+    for (int i = 0; i < kSymMultiVariants; ++i)
+        m = m > kSymVariants[i].r * sym_row_lds(kSymVariants[i]) ? m : kSymVariants[i].r * sym_row_lds(kSymVariants[i]);
+    return m;
+}
+constexpr int kNumBigLds = kNumVariants[kNumLdsVariants - 1].r * num_row_lds(kNumVariants[kNumLdsVariants - 1]);
+static_assert(max_sym_multi_lds() <= kSymMultiLds, "symbolic multi-bin LDS budget");
 
 // Slots of a global-memory hash table for a row with `keys` distinct keys.
 __host__ __device__ __forceinline__ long long global_table_slots(long long keys) {
@@ -85,12 +96,10 @@ template <typename V>
 void launch_sym_global(const ProbSet<V>& ps, const int2* fail_items, const long long* fail_off, int* pool,
                        int count, hipStream_t s);
 
+// fused pass: one launch per variant (0..9) over one bin's items
 template <typename V>
-void launch_symnum_multi(const ProbSet<V>& ps, const int2* items, const MultiArgs& ma, DevMeta* meta, int grid,
-                         hipStream_t s);
-template <typename V>
-void launch_symnum_big(const ProbSet<V>& ps, const int2* items, int item_off, int count, DevMeta* meta,
-                       int grid, hipStream_t s);
+hipError_t launch_symnum_bin(int variant, const ProbSet<V>& ps, const int2* items, int item_off, int count,
+                             DevMeta* meta, int grid, hipStream_t s);
 template <typename V>
 void launch_symnum_global(const ProbSet<V>& ps, const int2* items, const long long* gpool_off, int item_off,
                           int count, int* pool, DevMeta* meta, hipStream_t s);
@@ -101,8 +110,6 @@ int occupancy_sym_multi();
 template <typename V>
 int occupancy_sym_max();
 template <typename V>
-int occupancy_symnum_multi();
-template <typename V>
-int occupancy_symnum_big();
+int occupancy_symnum_bin(int variant);
 
 }  // namespace spg

@@ -1,0 +1,273 @@
+// Row-level building blocks shared by the symbolic and fused passes.
+//
+// A row of C is worked on by a group of L consecutive lanes (L <= 64: the
+// group lives inside one wave and synchronises at wave level; L > 64: the
+// group is the whole workgroup).  Its products a_ik * b_kj are enumerated
+// flat: the group loads the metadata of EC entries of A's row at once (B
+// row start and length, a_ik), takes an exclusive prefix of the B-row
+// lengths, and lane lr handles products p = lr, lr + L, ...; product p
+// belongs to the entry e with off[e] <= p < off[e+1], found by a binary
+// search in LDS.  Consecutive lanes read consecutive entries of a B row
+// (coalesced), every lane has up to PMAX independent loads in flight, and
+// short B rows leave no lane idle.
+#pragma once
+
+#include "spg_common.hpp"
+
+namespace spg {
+
+// Compiler-level ordering of LDS accesses among the lanes of one wave (a
+// wave's LDS instructions execute in order).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int L>
+__device__ __forceinline__ void gsync() {
+    if constexpr (L > kWave)
+        __syncthreads();
+    else
+        wave_sync();
+}
+
+// Exclusive prefix of v over the L lanes of a row group; `total` gets the
+// group sum.  For L > 64 the group is the workgroup and s_tmp holds one int
+// per wave.
+template <int L>
+__device__ __forceinline__ int group_excl_scan(int v, int lr, int* s_tmp, int& total) {
+    if constexpr (L == 1) {
+        total = v;
+        return 0;
+    } else if constexpr (L <= kWave) {
+        int incl = v;
+#pragma unroll
+        for (int o = 1; o < L; o <<= 1) {
+            const int y = __shfl_up(incl, o, L);
+            if (lr >= o) incl += y;
+        }
+        total = __shfl(incl, L - 1, L);
+        return incl - v;
+    } else {
+        constexpr int W = L / kWave;
+        const int lane = lr % kWave;
+        const int w = lr / kWave;
+        int incl = v;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
+        }
+        if (lane == kWave - 1) s_tmp[w] = incl;
+        __syncthreads();
+        int before = 0, all = 0;
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            const int s = s_tmp[i];
+            before += i < w ? s : 0;
+            all += s;
+        }
+        __syncthreads();
+        total = all;
+        return before + incl - v;
+    }
+}
+
+// Metadata of one chunk of EC entries of A's row, per row group (in LDS for
+// L > 1, in registers for a lane that owns its row).
+template <typename V, int EC, bool VALS>
+struct ChunkMeta {
+    int off[EC];             // exclusive prefix of B-row lengths (entries past the chunk: the chunk total)
+    int bs[EC];              // first position of the B row
+    V av[VALS ? EC : 1];     // a_ik
+};
+
+// Load entries [ebase, min(ebase + EC, a1)) of A's row into `cm`; returns
+// the chunk's product count.  Each lane loads EC/L consecutive entries.
+template <typename V, int L, int EC, bool VALS>
+__device__ __forceinline__ int load_chunk(const Prob<V>& P, int ebase, int a1, int lr,
+                                          ChunkMeta<V, EC, VALS>* cm, int* s_tmp) {
+    static_assert(EC % L == 0, "chunk must be a multiple of the group");
+    constexpr int PER = EC / L;
+    int len[PER], bs[PER];
+    V av[PER];
+    int local = 0;
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+        const int e = ebase + lr * PER + t;
+        len[t] = 0;
+        bs[t] = 0;
+        av[t] = V(0);
+        if (e < a1) {
+            const int k = P.a_col[e];
+            if constexpr (VALS) av[t] = P.a_val[e];
+            bs[t] = P.b_rpt[k];
+            len[t] = P.b_rpt[k + 1] - bs[t];
+        }
+        local += len[t];
+    }
+    int total;
+    int run = group_excl_scan<L>(local, lr, s_tmp, total);
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+        const int e = lr * PER + t;
+        cm->off[e] = run;
+        cm->bs[e] = bs[t];
+        if constexpr (VALS) cm->av[e] = av[t];
+        run += len[t];
+    }
+    gsync<L>();
+    return total;
+}
+
+// Entry of product p (0 <= p < total): the largest e with off[e] <= p.
+template <int EC>
+__device__ __forceinline__ int chunk_find(const int* off, int p) {
+    int e = 0;
+#pragma unroll
+    for (int step = EC / 2; step > 0; step >>= 1)
+        if (off[e + step] <= p) e += step;
+    return e;
+}
+
+// Fetch up to PMAX products of a chunk: slot j holds product
+// base + j*L + lr.  ok[j] tells which slots are real.
+template <typename V, int L, int EC, int PMAX, bool VALS>
+__device__ __forceinline__ void fetch_products(const Prob<V>& P, const ChunkMeta<V, EC, VALS>* cm, int total,
+                                               int base, int lr, int (&key)[PMAX], V (&prod)[PMAX],
+                                               bool (&ok)[PMAX]) {
+#pragma unroll
+    for (int j = 0; j < PMAX; ++j) {
+        const int p = base + j * L + lr;
+        ok[j] = p < total;
+        key[j] = 0;
+        prod[j] = V(0);
+        if (ok[j]) {
+            const int e = chunk_find<EC>(cm->off, p);
+            const int q = cm->bs[e] + p - cm->off[e];
+            key[j] = P.b_col[q];
+            if constexpr (VALS) prod[j] = cm->av[e] * P.b_val[q];
+        }
+    }
+}
+
+// A lane that owns its row keeps the chunk metadata in registers.
+template <typename V, int EC, bool VALS>
+__device__ __forceinline__ int load_chunk_lane(const Prob<V>& P, int ebase, int a1, ChunkMeta<V, EC, VALS>& cm) {
+    int run = 0;
+    int bs[EC], len[EC];
+#pragma unroll
+    for (int t = 0; t < EC; ++t) {
+        const int e = ebase + t;
+        bs[t] = 0;
+        len[t] = 0;
+        if constexpr (VALS) cm.av[t] = V(0);
+        if (e < a1) {
+            const int k = P.a_col[e];
+            if constexpr (VALS) cm.av[t] = P.a_val[e];
+            bs[t] = P.b_rpt[k];
+            len[t] = P.b_rpt[k + 1] - bs[t];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < EC; ++t) {
+        cm.off[t] = run;
+        cm.bs[t] = bs[t];
+        run += len[t];
+    }
+    return run;
+}
+
+template <typename V, int EC, int PMAX, bool VALS>
+__device__ __forceinline__ void fetch_products_lane(const Prob<V>& P, const ChunkMeta<V, EC, VALS>& cm, int total,
+                                                    int base, int (&key)[PMAX], V (&prod)[PMAX], bool (&ok)[PMAX]) {
+#pragma unroll
+    for (int j = 0; j < PMAX; ++j) {
+        const int p = base + j;
+        ok[j] = p < total;
+        key[j] = 0;
+        prod[j] = V(0);
+        if (ok[j]) {
+            int off = cm.off[0], bs = cm.bs[0];
+            V av = VALS ? cm.av[0] : V(0);
+#pragma unroll
+            for (int t = 1; t < EC; ++t) {
+                if (cm.off[t] <= p) {
+                    off = cm.off[t];
+                    bs = cm.bs[t];
+                    if constexpr (VALS) av = cm.av[t];
+                }
+            }
+            const int q = bs + p - off;
+            key[j] = P.b_col[q];
+            if constexpr (VALS) prod[j] = av * P.b_val[q];
+        }
+    }
+}
+
+// Call f(key, prod) for every product of A's entries [a0, a1).
+template <typename V, int L, int EC, int PMAX, bool VALS, typename F>
+__device__ __forceinline__ void for_each_product(const Prob<V>& P, int a0, int a1, int lr,
+                                                 ChunkMeta<V, EC, VALS>* cm, int* s_tmp, F&& f) {
+    for (int ebase = a0; ebase < a1; ebase += EC) {
+        int key[PMAX];
+        V prod[PMAX];
+        bool ok[PMAX];
+        if constexpr (L == 1) {
+            ChunkMeta<V, EC, VALS> rm;
+            const int total = load_chunk_lane<V, EC, VALS>(P, ebase, a1, rm);
+            for (int base = 0; base < total; base += PMAX) {
+                fetch_products_lane<V, EC, PMAX, VALS>(P, rm, total, base, key, prod, ok);
+#pragma unroll
+                for (int j = 0; j < PMAX; ++j)
+                    if (ok[j]) f(key[j], prod[j]);
+            }
+        } else {
+            const int total = load_chunk<V, L, EC, VALS>(P, ebase, a1, lr, cm, s_tmp);
+            for (int base = 0; base < total; base += L * PMAX) {
+                fetch_products<V, L, EC, PMAX, VALS>(P, cm, total, base, lr, key, prod, ok);
+#pragma unroll
+                for (int j = 0; j < PMAX; ++j)
+                    if (ok[j]) f(key[j], prod[j]);
+            }
+            gsync<L>();  // the chunk metadata is overwritten next
+        }
+    }
+}
+
+// Fill n ints of LDS with v using 16-byte stores where possible (p 16-byte aligned, n % 4 == 0).
+template <int N, int L>
+__device__ __forceinline__ void lds_fill(int* p, int v, int lr) {
+    if constexpr (N % 4 == 0 && N >= 4 * L) {
+        int4* q = reinterpret_cast<int4*>(p);
+        const int4 w = make_int4(v, v, v, v);
+#pragma unroll 4
+        for (int j = lr; j < N / 4; j += L) q[j] = w;
+    } else {
+        for (int j = lr; j < N; j += L) p[j] = v;
+    }
+}
+
+// Ascending bitonic sort of P ints (P a power of two) in LDS by the L lanes
+// of a row group.
+template <int P, int L>
+__device__ __forceinline__ void bitonic_sort(int* cols, int lr) {
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = lr; t < P / 2; t += L) {
+                const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+                const int a = cols[i];
+                const int b = cols[i + j];
+                const bool asc = (i & k) == 0;
+                if ((a > b) == asc) {
+                    cols[i] = b;
+                    cols[i + j] = a;
+                }
+            }
+            gsync<L>();
+        }
+    }
+}
+
+}  // namespace spg

@@ -1,0 +1,190 @@
+// Thread-per-lane execution of emulated kernel launches (see
+// tests/emu/include/hip/hip_runtime.h).  Blocks run one after another on a
+// pool of worker threads; barriers count live lanes, so lanes that have
+// finished the kernel no longer hold others back.
+#include <hip/hip_runtime.h>
+
+#include <condition_variable>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+namespace emu {
+
+thread_local Idx tl_thread, tl_block;
+dim3 g_block_dim, g_grid_dim;
+
+namespace {
+
+// Lanes parked at the workgroup barrier count as present for their wave's
+// barriers: on hardware they are masked off while the rest of the wave
+// runs on, so they never hold the wave back.
+struct Barrier {
+    std::mutex m;
+    std::condition_variable cv;
+    int live = 0;
+    int arrived = 0;
+    int parked = 0;
+    unsigned long long gen = 0;
+
+    void reset(int n) {
+        std::lock_guard<std::mutex> g(m);
+        live = n;
+        arrived = 0;
+        parked = 0;
+    }
+    void release_if_complete() {  // m held
+        if (arrived > 0 && arrived + parked == live) {
+            arrived = 0;
+            ++gen;
+            cv.notify_all();
+        }
+    }
+    template <typename OnRelease>
+    void arrive(OnRelease on_release) {
+        std::unique_lock<std::mutex> g(m);
+        const unsigned long long my = gen;
+        ++arrived;
+        if (arrived + parked == live) on_release();
+        release_if_complete();
+        if (gen != my) return;
+        cv.wait(g, [&] { return gen != my; });
+    }
+    void arrive() {
+        arrive([] {});
+    }
+    void leave() {
+        std::lock_guard<std::mutex> g(m);
+        --live;
+        release_if_complete();
+    }
+    void park(int d) {
+        std::lock_guard<std::mutex> g(m);
+        parked += d;
+        release_if_complete();
+    }
+};
+
+constexpr int kMaxThreads = 1024;
+Barrier g_block_bar;
+Barrier g_wave_bar[kMaxThreads / 64];
+std::uint64_t g_slots[kMaxThreads];
+
+// The last lane to reach the workgroup barrier un-parks every wave before
+// releasing it (all live lanes are at the barrier at that moment).
+static void unpark_all() {
+    for (auto& w : g_wave_bar) {
+        std::lock_guard<std::mutex> g(w.m);
+        w.parked = 0;
+    }
+}
+
+struct Pool {
+    std::vector<std::thread> workers;
+    std::mutex m;
+    std::condition_variable cv_start, cv_done;
+    const std::function<void()>* body = nullptr;
+    unsigned block = 0;
+    int active = 0;       // threads taking part in the current block
+    int remaining = 0;    // threads still running it
+    unsigned long long round = 0;
+    bool stop = false;
+
+    void ensure(int n) {
+        while (static_cast<int>(workers.size()) < n) {
+            const int id = static_cast<int>(workers.size());
+            workers.emplace_back([this, id] { loop(id); });
+        }
+    }
+    void loop(int id) {
+        unsigned long long seen = 0;
+        for (;;) {
+            const std::function<void()>* fn;
+            unsigned blk;
+            {
+                std::unique_lock<std::mutex> g(m);
+                cv_start.wait(g, [&] { return stop || (round != seen && id < active); });
+                if (stop) return;
+                seen = round;
+                fn = body;
+                blk = block;
+            }
+            tl_thread = Idx{static_cast<unsigned>(id), 0, 0};
+            tl_block = Idx{blk, 0, 0};
+            (*fn)();
+            g_wave_bar[id / 64].leave();
+            {
+                std::lock_guard<std::mutex> g(g_block_bar.m);
+                --g_block_bar.live;
+                if (g_block_bar.arrived > 0 && g_block_bar.arrived + g_block_bar.parked == g_block_bar.live) {
+                    unpark_all();
+                    g_block_bar.release_if_complete();
+                }
+            }
+            {
+                std::lock_guard<std::mutex> g(m);
+                if (--remaining == 0) cv_done.notify_all();
+            }
+        }
+    }
+    ~Pool() {
+        {
+            std::lock_guard<std::mutex> g(m);
+            stop = true;
+        }
+        cv_start.notify_all();
+        for (auto& t : workers) t.join();
+    }
+};
+
+Pool& pool() {
+    static Pool p;
+    return p;
+}
+
+}  // namespace
+
+int lane() { return static_cast<int>(tl_thread.x % 64); }
+void block_barrier() {
+    g_wave_bar[tl_thread.x / 64].park(1);
+    g_block_bar.arrive(unpark_all);
+}
+void wave_barrier() { g_wave_bar[tl_thread.x / 64].arrive(); }
+void exchange_write(std::uint64_t v) { g_slots[tl_thread.x] = v; }
+std::uint64_t exchange_read(int l) { return g_slots[(tl_thread.x / 64) * 64 + static_cast<unsigned>(l)]; }
+
+void launch(dim3 grid, dim3 block, const std::function<void()>& body) {
+    const int n = static_cast<int>(block.x);
+    if (n < 1 || n > kMaxThreads || block.y != 1 || block.z != 1 || grid.y != 1 || grid.z != 1) std::abort();
+    Pool& p = pool();
+    p.ensure(n);
+    g_block_dim = block;
+    g_grid_dim = grid;
+    for (unsigned b = 0; b < grid.x; ++b) {
+        g_block_bar.reset(n);
+        for (int w = 0; w < (n + 63) / 64; ++w) g_wave_bar[w].reset(std::min(64, n - 64 * w));
+        {
+            std::lock_guard<std::mutex> g(p.m);
+            p.body = &body;
+            p.block = b;
+            p.active = n;
+            p.remaining = n;
+            ++p.round;
+        }
+        p.cv_start.notify_all();
+        std::unique_lock<std::mutex> g(p.m);
+        p.cv_done.wait(g, [&] { return p.remaining == 0; });
+    }
+}
+
+}  // namespace emu
+
+hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int) {
+    std::memset(p, 0, sizeof(*p));
+    std::strcpy(p->gcnArchName, "gfx950:emulated");
+    p->multiProcessorCount = 8;
+    p->sharedMemPerBlock = 163840;
+    p->clockRate = 2400000;
+    p->totalGlobalMem = std::size_t(1) << 34;
+    return hipSuccess;
+}

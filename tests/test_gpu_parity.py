@@ -165,25 +165,26 @@ def _mixed_case():
     return A, A
 
 
-SYM_VARIANT_T = [16, 32, 64, 128, 256, 512, 2048, 8192]
-NUM_VARIANT_P = [8, 16, 32, 64, 128, 128, 512, 1024, 2048]
+SYM_VARIANT_T = [16, 32, 64, 128, 256, 512, 1024, 4096, 8192]   # multi-bin symbolic variants
+SYM_MAX_VARIANT = 9
+NUM_VARIANT_P = [8, 16, 32, 64, 128, 128, 512, 1024, 2048, 8192]  # fused LDS variants
+NUM_GLOBAL_VARIANT = 10
 
 
-@pytest.mark.parametrize("variant", range(8))
+@pytest.mark.parametrize("variant", range(len(SYM_VARIANT_T)))
 def test_every_symbolic_variant(gpu_lib, variant):
     A, B = _mixed_case()
-    g.set_tuning(0, f"{variant} {SYM_VARIANT_T[variant] // 2} {INT_MAX}\n8 {INT_MAX} {INT_MAX}\n")
+    g.set_tuning(0, f"{variant} {SYM_VARIANT_T[variant] // 2} {INT_MAX}\n{SYM_MAX_VARIANT} {INT_MAX} {INT_MAX}\n")
     try:
         compare(A, B, gpu_multiply(A, B))
     finally:
         g.set_tuning(0, None)
 
 
-@pytest.mark.parametrize("variant", range(10))
+@pytest.mark.parametrize("variant", range(len(NUM_VARIANT_P)))
 def test_every_fused_variant(gpu_lib, variant):
     A, B = _mixed_case()
-    cap = NUM_VARIANT_P[variant] if variant < 9 else 8192
-    g.set_tuning(1, f"{variant} {INT_MAX} {cap}\n10 {INT_MAX} {INT_MAX}\n")
+    g.set_tuning(1, f"{variant} {INT_MAX} {NUM_VARIANT_P[variant]}\n{NUM_GLOBAL_VARIANT} {INT_MAX} {INT_MAX}\n")
     try:
         compare(A, B, gpu_multiply(A, B))
     finally:
@@ -194,8 +195,8 @@ def test_global_paths_forced(gpu_lib):
     """Every row through the largest-table and global kernels."""
     A = gen.random_csr(600, 800, 0.02, seed=31)
     B = gen.random_csr(800, 700, 0.02, seed=32)
-    g.set_tuning(0, f"8 {INT_MAX} {INT_MAX}\n")
-    g.set_tuning(1, f"10 {INT_MAX} {INT_MAX}\n")
+    g.set_tuning(0, f"{SYM_MAX_VARIANT} {INT_MAX} {INT_MAX}\n")
+    g.set_tuning(1, f"{NUM_GLOBAL_VARIANT} {INT_MAX} {INT_MAX}\n")
     try:
         compare(A, B, gpu_multiply(A, B))
     finally:
