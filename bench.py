@@ -189,6 +189,25 @@ def cpu_baseline(A, nip: int, budget_s: float) -> dict:
                       f"({threads} threads), {t_total:.1f} s"}
 
 
+def copy_calibration(mbytes: int, reps: int = 5) -> None:
+    """Device-to-device copies of a known size, for scaling rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters to bytes (run under --pmc)."""
+    import ctypes
+    import gpuspgemm_amd as g
+    from gpuspgemm_amd import _lib
+    lib = g.load_library()
+    n = mbytes << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    _lib.check(lib.spg_malloc(ctypes.byref(src), n))
+    _lib.check(lib.spg_malloc(ctypes.byref(dst), n))
+    for _ in range(reps):
+        _lib.check(lib.spg_memcpy(dst, src, n))
+    _lib.check(lib.spg_device_sync())
+    lib.spg_free(src)
+    lib.spg_free(dst)
+    print(json.dumps({"copy_calibration_bytes": n, "reps": reps}), flush=True)
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,7 +219,12 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="only run device copies of MB MiB (counter calibration)")
     args = ap.parse_args(argv)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the library is single-GPU, "
               "see DESIGN.md for how it would shard)", file=sys.stderr)

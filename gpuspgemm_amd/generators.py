@@ -111,13 +111,15 @@ def _counter(rnd: int, rows: np.ndarray, j: np.ndarray, m: int) -> np.ndarray:
     return ((np.int64(rnd) * m + rows) << 21) + j
 
 
-def uniform8(m: int = 4096, per_row: int = 8, seed: int = 0) -> HostCSR:
-    """m x m, exactly `per_row` distinct uniform columns per row, values U[-1,1]."""
+def uniform8(m: int = 4096, per_row: int = 8, seed: int = 0, n: int | None = None) -> HostCSR:
+    """m x n (n = m by default), exactly `per_row` distinct uniform columns
+    per row, values U[-1,1]."""
+    n = m if n is None else n
     def draw(rnd, rows, j):
-        return rand_below(seed, 1, _counter(rnd, rows, j, m), np.full(rows.shape, m))
-    rpt, col = _distinct_rows(seed, m, m, np.full(m, per_row), draw)
+        return rand_below(seed, 1, _counter(rnd, rows, j, m), np.full(rows.shape, n))
+    rpt, col = _distinct_rows(seed, m, n, np.full(m, per_row), draw)
     val = rand_unit(seed, 2, np.arange(col.size)) * 2.0 - 1.0
-    return HostCSR(m, m, rpt, col, val)
+    return HostCSR(m, n, rpt, col, val)
 
 
 def powerlaw(m: int = 1_000_000, seed: int = 1, max_len: int = 20000,

@@ -1,8 +1,10 @@
 """The library's kernels and host driver, compiled as host C++ against the
 CPU emulation of the HIP API in tests/emu (one thread per GPU lane) and run
-under AddressSanitizer on small inputs: every bin variant, the global
-fallbacks, and out-of-bounds accesses caught on the CPU.  Results are
-checked against the host Gustavson product by tests/emu/emu_main.cpp."""
+under AddressSanitizer: every bin variant with rows that fit it in one
+chunk and rows spread over several chunks, the global fallbacks, and
+out-of-bounds accesses caught on the CPU.  Results are checked against the
+host Gustavson product by tests/emu/emu_main.cpp.  Inputs are small:
+emulating a 1024-lane workgroup costs a thread per lane."""
 import os
 import subprocess
 
@@ -18,6 +20,15 @@ BUILD = os.path.join(EMU, "_build")
 EXE = os.path.join(BUILD, "emu_spgemm")
 CSRC = os.path.join(ROOT, "gpuspgemm_amd", "csrc")
 INT_MAX = 2**31 - 1
+
+# (lanes per row, A entries per chunk, table slots) of the symbolic variants
+SYM = [(1, 4, 16), (2, 4, 32), (4, 8, 64), (8, 16, 128), (16, 32, 256), (64, 64, 512), (64, 64, 1024),
+       (256, 256, 4096), (256, 256, 8192)]
+SYM_MAX = 9
+# (lanes per row, A entries per chunk, capacity) of the fused variants
+NUM = [(1, 4, 8), (2, 4, 16), (4, 8, 32), (8, 16, 64), (16, 32, 128), (64, 64, 128), (64, 64, 512),
+       (256, 256, 1024), (256, 256, 2048), (1024, 1024, 8192)]
+NUM_GLOBAL = 10
 
 
 @pytest.fixture(scope="module")
@@ -47,17 +58,29 @@ def run_emu(exe, tmp_path, A, B, *extra, tables=None):
     return res.stdout
 
 
-def test_uniform_and_stencil(emu, tmp_path):
+def squares(m, k, seed):
+    """k entries per row of A and of B: NIP = k^2, nnz <= k^2."""
+    n = max(4 * k, 64)
+    return gen.uniform8(m=m, per_row=k, seed=seed, n=n), gen.uniform8(m=n, per_row=k, seed=seed + 1, n=40 * k)
+
+
+def chunked(m, k, seed):
+    """k entries per row of A, one per row of B: NIP = nnz <= k, A rows span k/EC chunks."""
+    n = max(4 * k, 64)
+    return gen.uniform8(m=m, per_row=k, seed=seed, n=n), gen.uniform8(m=n, per_row=1, seed=seed + 1, n=n)
+
+
+def rows_for(lanes):
+    return 96 if lanes <= 16 else (12 if lanes <= 256 else 3)
+
+
+def test_uniform_stencil_powerlaw(emu, tmp_path):
     A = gen.uniform8(m=512)
     run_emu(emu, tmp_path, A, A)
-    S = gen.stencil27(g=6)
-    run_emu(emu, tmp_path, S, S)
+    S = gen.stencil27(g=5)
     run_emu(emu, tmp_path, S.astype(np.float32), S.astype(np.float32), "--f32")
-
-
-def test_powerlaw_mixed_bins(emu, tmp_path):
-    A = gen.powerlaw(m=4000, seed=5)
-    run_emu(emu, tmp_path, A, A, "--repeat", "2")
+    P = gen.powerlaw(m=1500, seed=5)
+    run_emu(emu, tmp_path, P, P, "--repeat", "2")
 
 
 def test_rectangular_and_empty_rows(emu, tmp_path):
@@ -66,30 +89,33 @@ def test_rectangular_and_empty_rows(emu, tmp_path):
     run_emu(emu, tmp_path, A, B)
 
 
-SYM_T = [16, 32, 64, 128, 256, 512, 1024, 4096, 8192]
-NUM_P = [8, 16, 32, 64, 128, 128, 512, 1024, 2048, 8192]
-
-
-@pytest.mark.parametrize("variant", range(len(SYM_T)))
+@pytest.mark.parametrize("variant", range(len(SYM)))
 def test_symbolic_variant(emu, tmp_path, variant):
-    A = gen.powerlaw(m=1500, seed=7)
-    table = f"{variant} {SYM_T[variant] // 2} {INT_MAX}\n9 {INT_MAX} {INT_MAX}\n"
-    run_emu(emu, tmp_path, A, A, tables={0: table})
+    lanes, ec, t = SYM[variant]
+    cap = t // 2
+    table = {0: f"{variant} {cap} {INT_MAX}\n{SYM_MAX} {INT_MAX} {INT_MAX}\n"}
+    m = rows_for(lanes)
+    k = int(np.sqrt(cap))
+    run_emu(emu, tmp_path, *squares(m, k, seed=variant), tables=table)
+    run_emu(emu, tmp_path, *chunked(m, min(cap, 2 * ec + 1), seed=variant), tables=table)
 
 
-@pytest.mark.parametrize("variant", range(len(NUM_P)))
+@pytest.mark.parametrize("variant", range(len(NUM)))
 def test_fused_variant(emu, tmp_path, variant):
-    A = gen.powerlaw(m=1500, seed=7)
-    table = f"{variant} {INT_MAX} {NUM_P[variant]}\n10 {INT_MAX} {INT_MAX}\n"
-    run_emu(emu, tmp_path, A, A, tables={1: table})
+    lanes, ec, p = NUM[variant]
+    table = {1: f"{variant} {INT_MAX} {p}\n{NUM_GLOBAL} {INT_MAX} {INT_MAX}\n"}
+    m = rows_for(lanes)
+    run_emu(emu, tmp_path, *squares(m, int(np.sqrt(p)), seed=10 + variant), tables=table)
+    run_emu(emu, tmp_path, *chunked(m, min(p, 2 * ec + 1), seed=10 + variant), "--f32", tables=table)
 
 
 def test_global_paths(emu, tmp_path):
-    """Every row through the largest-table and global-memory kernels,
-    including a row that overflows the largest LDS table."""
-    nb = 40_000
-    r0 = np.unique(gen.rand_below(3, 1, np.arange(12000), np.full(12000, nb)))
-    rows = [r0] + [np.unique(gen.rand_below(3, 3 + i, np.arange(3), np.full(3, nb))) for i in range(20)]
+    """A row that overflows the largest LDS table in the count pass and every
+    LDS capacity in the fused pass, plus ordinary rows, first with the
+    default tables and then with every row forced to the global kernels."""
+    nb = 20_000
+    r0 = np.unique(gen.rand_below(3, 1, np.arange(9500), np.full(9500, nb)))
+    rows = [r0] + [np.unique(gen.rand_below(3, 3 + i, np.arange(3), np.full(3, nb))) for i in range(3)]
     rpt = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
     col = np.concatenate(rows).astype(np.int32)
     A = HostCSR(len(rows), nb, rpt, col, gen.rand_unit(3, 9, np.arange(col.size)) * 2 - 1)
@@ -99,5 +125,6 @@ def test_global_paths(emu, tmp_path):
     np.cumsum(np.bincount(keys // 200_000, minlength=nb), out=brpt[1:])
     bcol = (keys % 200_000).astype(np.int32)
     B = HostCSR(nb, 200_000, brpt.astype(np.int32), bcol, gen.rand_unit(4, 2, np.arange(bcol.size)) * 2 - 1)
-    run_emu(emu, tmp_path, A, B, tables={0: f"9 {INT_MAX} {INT_MAX}\n", 1: f"10 {INT_MAX} {INT_MAX}\n"})
     run_emu(emu, tmp_path, A, B)
+    run_emu(emu, tmp_path, A, B,
+            tables={0: f"{SYM_MAX} {INT_MAX} {INT_MAX}\n", 1: f"{NUM_GLOBAL} {INT_MAX} {INT_MAX}\n"})
