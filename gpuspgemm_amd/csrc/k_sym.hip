@@ -17,10 +17,11 @@ __device__ __forceinline__ void sym_rows(const ProbSet<V>& ps, const int2* __res
     static_assert(WG % R == 0, "bad variant shape");
     static_assert(L <= kWave || R == 1, "a row spans several waves only when it owns the workgroup");
     using Meta = ChunkMeta<V, EC, false>;
+    constexpr int ROW_INTS = sym_row_ints(SymVariant{WG, R, T, EC, PMAX});
     const int slot = threadIdx.x / L;
     const int lr = threadIdx.x % L;
-    int* tab = smem + slot * T;
-    Meta* cm = reinterpret_cast<Meta*>(smem + R * T) + slot;
+    int* tab = smem + slot * ROW_INTS;
+    Meta* cm = reinterpret_cast<Meta*>(tab + T);
     lds_fill<T, L>(tab, kEmpty, lr);
     gsync<L>();
 
@@ -34,8 +35,9 @@ __device__ __forceinline__ void sym_rows(const ProbSet<V>& ps, const int2* __res
         a1 = P.a_rpt[it.y + 1];
     }
     int mine = 0;
-    for_each_product<V, L, EC, PMAX, false>(P, a0, a1, lr, cm, s_tmp,
-                                            [&](int key, V) { mine += table_insert<T>(tab, key); });
+    for_each_round<V, L, EC, PMAX, false>(
+        P, a0, a1, lr, cm, s_tmp,
+        [&](const int(&key)[PMAX], const V(&)[PMAX], const bool(&ok)[PMAX]) { mine += table_insert_all<T>(tab, key, ok); });
     int nnz;
     if constexpr (L <= kWave) {
         nnz = group_sum<L>(mine);
@@ -143,10 +145,7 @@ __global__ __launch_bounds__(1024) void k_sym_max(ProbSet<V> ps, const int2* __r
                 V prod[PMAX];
                 bool ok[PMAX];
                 fetch_products<V, WG, EC, PMAX, false>(P, &cm, total, base, threadIdx.x, key, prod, ok);
-                int mine = 0;
-#pragma unroll
-                for (int j = 0; j < PMAX; ++j)
-                    if (ok[j]) mine += table_insert<T>(tab, key[j]);
+                const int mine = table_insert_all<T>(tab, key, ok);
                 if (mine) atomicAdd(&s_n, mine);
                 __syncthreads();
                 over = s_n > kSymMaxThreshold;  // uniform: every lane reads after the barrier

@@ -49,16 +49,28 @@ __device__ __forceinline__ int compact_table(const int* tab, int* cols, int lr, 
     return n;
 }
 
+// Ascending list of the occupied slots of a row's table in cols[0..n),
+// padded with INT_MAX up to P; returns the number of occupied slots.
+template <int L, int T, int P>
+__device__ __forceinline__ int sort_row(const int* tab, int* cols, int lr, int* s_tmp) {
+    const int found = compact_table<L, T, P>(tab, cols, lr, s_tmp);
+    const int n = min(found, P);
+    for (int j = n + lr; j < P; j += L) cols[j] = INT_MAX;
+    gsync<L>();
+    bitonic_sort<P, L>(cols, lr);
+    return found;
+}
+
 template <typename V, int WG, int R, int P, int EC, int PMAX>
 __device__ __forceinline__ void symnum_rows(const ProbSet<V>& ps, const int2* __restrict__ items, int first,
                                             int end, int* smem, int* s_tmp, DevMeta* meta) {
     constexpr int L = WG / R;
     constexpr int T = 2 * P;
-    constexpr int ROW_INTS = num_row_lds(NumVariant{WG, R, P, EC, PMAX}) / 4;
+    constexpr int ROW_INTS = num_row_ints(NumVariant{WG, R, P, EC, PMAX});
     static_assert(WG % R == 0, "bad variant shape");
     static_assert(L <= kWave || R == 1, "a row spans several waves only when it owns the workgroup");
     static_assert(T * sizeof(int) >= P * sizeof(V), "values must fit in the table's LDS");
-    static_assert(ROW_INTS % 4 == 0, "row slots must stay 16-byte aligned");
+    static_assert(ROW_INTS % 2 == 0, "row slots must stay 8-byte aligned");
     using Meta = ChunkMeta<V, EC, true>;
     using KeyMeta = ChunkMeta<V, EC, false>;
     const int slot = threadIdx.x / L;
@@ -96,20 +108,16 @@ __device__ __forceinline__ void symnum_rows(const ProbSet<V>& ps, const int2* __
             total = load_chunk<V, L, EC, true>(Pr, a0, a1, lr, cm, s_tmp);
             fetch_products<V, L, EC, PMAX, true>(Pr, cm, total, 0, lr, key, prod, ok);
         }
-#pragma unroll
-        for (int j = 0; j < PMAX; ++j)
-            if (ok[j]) table_insert<T>(tab, key[j]);
+        table_insert_all<T>(tab, key, ok);
     } else {
-        for_each_product<V, L, EC, PMAX, false>(Pr, a0, a1, lr, reinterpret_cast<KeyMeta*>(cm), s_tmp,
-                                                [&](int k, V) { table_insert<T>(tab, k); });
+        for_each_round<V, L, EC, PMAX, false>(
+            Pr, a0, a1, lr, reinterpret_cast<KeyMeta*>(cm), s_tmp,
+            [&](const int(&k)[PMAX], const V(&)[PMAX], const bool(&o)[PMAX]) { table_insert_all<T>(tab, k, o); });
     }
     gsync<L>();
 
-    const int found = compact_table<L, T, P>(tab, cols, lr, s_tmp);
+    const int found = sort_row<L, T, P>(tab, cols, lr, s_tmp);
     const int n = min(found, P);
-    for (int j = n + lr; j < P; j += L) cols[j] = INT_MAX;
-    gsync<L>();
-    bitonic_sort<P, L>(cols, lr);
 
     int c0 = 0, w = 0;
     if (active) {
@@ -147,7 +155,7 @@ template <typename V, int VV>
 __global__ __launch_bounds__(kNumVariants[VV].wg) void k_symnum_bin(ProbSet<V> ps, const int2* __restrict__ items,
                                                                     int item_off, int count, DevMeta* meta) {
     constexpr NumVariant kV = kNumVariants[VV];
-    __shared__ __align__(16) int smem[kV.r * num_row_lds(kV) / 4];
+    __shared__ __align__(16) int smem[kV.r * num_row_ints(kV)];
     __shared__ int s_tmp[kV.wg / kWave];
     const int units = (count + kV.r - 1) / kV.r;
     const UnitRange ur = xcd_units(units);

@@ -153,14 +153,13 @@ __device__ __forceinline__ int lds_load(const int* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Open-addressing insert with linear probing into an LDS table of T slots.
+// Open-addressing insert with linear probing into an LDS table of T slots,
+// continuing from slot h whose content `cur` has already been read.
 // Returns 1 when the key was not present.  The table never fills: callers
 // size T above the number of distinct keys of the row.
 template <int T>
-__device__ __forceinline__ int table_insert(int* tab, int key) {
-    uint32_t h = hash_slot<T>(key);
+__device__ __forceinline__ int table_insert_from(int* tab, int key, uint32_t h, int cur) {
     while (true) {
-        const int cur = lds_load(tab + h);
         if (cur == key) return 0;
         if (cur == kEmpty) {
             const int old = atomicCAS(tab + h, kEmpty, key);
@@ -168,7 +167,32 @@ __device__ __forceinline__ int table_insert(int* tab, int key) {
             if (old == key) return 0;
         }
         h = (h + 1) & (T - 1);
+        cur = lds_load(tab + h);
     }
+}
+
+template <int T>
+__device__ __forceinline__ int table_insert(int* tab, int key) {
+    const uint32_t h = hash_slot<T>(key);
+    return table_insert_from<T>(tab, key, h, lds_load(tab + h));
+}
+
+// Insert up to N staged keys: the first probes of all keys are read
+// together (independent LDS loads), then each key finishes its probe.
+template <int T, int N>
+__device__ __forceinline__ int table_insert_all(int* tab, const int (&key)[N], const bool (&ok)[N]) {
+    uint32_t h[N];
+    int cur[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        h[j] = hash_slot<T>(key[j]);
+        cur[j] = ok[j] ? lds_load(tab + h[j]) : key[j];
+    }
+    int added = 0;
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+        if (ok[j] && cur[j] != key[j]) added += table_insert_from<T>(tab, key[j], h[j], cur[j]);
+    return added;
 }
 
 // Sum over the L consecutive lanes of a row group (L a power of two <= 64).

@@ -30,7 +30,7 @@ constexpr SymVariant kSymVariants[] = {
 constexpr int kSymMultiVariants = 9;
 constexpr int kSymMaxVariant = 9;
 constexpr int kSymMaxThreshold = 26214;  // distinct keys admitted in the 32768-slot table (80 %)
-constexpr int kSymMultiLds = 36 * 1024;  // bytes: max over variants 0..8 of r * (4t + 8ec + 4)
+constexpr int kSymMultiLds = 36 * 1024;  // bytes: max over variants 0..8 of r * sym_row_ints * 4
 
 // Fused sorted-columns + numeric variants, one kernel each: 0..8 use 256
 // threads, 9 a 1024-thread workgroup per row; rows above its capacity use
@@ -46,17 +46,33 @@ constexpr int kNumGlobalVariant = 10;
 
 constexpr int kGlobalSortChunk = 4096;  // LDS chunk of the global-bin column sort
 
-// LDS bytes of one row slot of a fused variant (FP64 sizing): table 2p
-// ints (reused for p values), sorted list p ints, chunk metadata.
-constexpr int num_row_lds(const NumVariant& v) { return 12 * v.p + (v.wg / v.r > 1 ? 16 * v.ec : 0); }
-constexpr int sym_row_lds(const SymVariant& v) { return 4 * v.t + (v.wg / v.r > 1 ? 8 * v.ec + 8 : 0); }
+// Ints of one row slot in LDS: `bytes` rounded up to 8-byte words and
+// padded so that the number of words is odd.  Slots of one wave then start
+// in different LDS banks; with a power-of-two stride the same offset of
+// every row of the wave falls in one bank, and the small-row variants (up
+// to 64 rows per wave) serialise on bank conflicts.
+constexpr int padded_row_ints(int bytes) {
+    const int words = (bytes + 7) / 8;
+    return 2 * (words % 2 ? words : words + 1);
+}
+// Fused variant (FP64 sizing): table 2p ints (reused for p values), sorted
+// list p ints, chunk metadata when the row has more than one lane.
+constexpr int num_row_ints(const NumVariant& v) {
+    return padded_row_ints(12 * v.p + (v.wg / v.r > 1 ? 16 * v.ec : 0));
+}
+// Symbolic variant: table t ints, chunk metadata without values.
+constexpr int sym_row_ints(const SymVariant& v) {
+    return padded_row_ints(4 * v.t + (v.wg / v.r > 1 ? 8 * v.ec + 8 : 0));
+}
 constexpr int max_sym_multi_lds() {
     int m = 0;
-    for (int i = 0; i < kSymMultiVariants; ++i)
-        m = m > kSymVariants[i].r * sym_row_lds(kSymVariants[i]) ? m : kSymVariants[i].r * sym_row_lds(kSymVariants[i]);
+    for (int i = 0; i < kSymMultiVariants; ++i) {
+        const int b = 4 * kSymVariants[i].r * sym_row_ints(kSymVariants[i]);
+        m = m > b ? m : b;
+    }
     return m;
 }
-constexpr int kNumBigLds = kNumVariants[kNumLdsVariants - 1].r * num_row_lds(kNumVariants[kNumLdsVariants - 1]);
+constexpr int kNumBigLds = 4 * kNumVariants[kNumLdsVariants - 1].r * num_row_ints(kNumVariants[kNumLdsVariants - 1]);
 static_assert(max_sym_multi_lds() <= kSymMultiLds, "symbolic multi-bin LDS budget");
 
 // Slots of a global-memory hash table for a row with `keys` distinct keys.

@@ -206,10 +206,11 @@ __device__ __forceinline__ void fetch_products_lane(const Prob<V>& P, const Chun
     }
 }
 
-// Call f(key, prod) for every product of A's entries [a0, a1).
-template <typename V, int L, int EC, int PMAX, bool VALS, typename F>
-__device__ __forceinline__ void for_each_product(const Prob<V>& P, int a0, int a1, int lr,
-                                                 ChunkMeta<V, EC, VALS>* cm, int* s_tmp, F&& f) {
+// Call g(key[], prod[], ok[]) for every round of up to PMAX products per
+// lane of A's entries [a0, a1).
+template <typename V, int L, int EC, int PMAX, bool VALS, typename G>
+__device__ __forceinline__ void for_each_round(const Prob<V>& P, int a0, int a1, int lr,
+                                               ChunkMeta<V, EC, VALS>* cm, int* s_tmp, G&& g) {
     for (int ebase = a0; ebase < a1; ebase += EC) {
         int key[PMAX];
         V prod[PMAX];
@@ -219,31 +220,39 @@ __device__ __forceinline__ void for_each_product(const Prob<V>& P, int a0, int a
             const int total = load_chunk_lane<V, EC, VALS>(P, ebase, a1, rm);
             for (int base = 0; base < total; base += PMAX) {
                 fetch_products_lane<V, EC, PMAX, VALS>(P, rm, total, base, key, prod, ok);
-#pragma unroll
-                for (int j = 0; j < PMAX; ++j)
-                    if (ok[j]) f(key[j], prod[j]);
+                g(key, prod, ok);
             }
         } else {
             const int total = load_chunk<V, L, EC, VALS>(P, ebase, a1, lr, cm, s_tmp);
             for (int base = 0; base < total; base += L * PMAX) {
                 fetch_products<V, L, EC, PMAX, VALS>(P, cm, total, base, lr, key, prod, ok);
-#pragma unroll
-                for (int j = 0; j < PMAX; ++j)
-                    if (ok[j]) f(key[j], prod[j]);
+                g(key, prod, ok);
             }
             gsync<L>();  // the chunk metadata is overwritten next
         }
     }
 }
 
-// Fill n ints of LDS with v using 16-byte stores where possible (p 16-byte aligned, n % 4 == 0).
+// Call f(key, prod) for every product of A's entries [a0, a1).
+template <typename V, int L, int EC, int PMAX, bool VALS, typename F>
+__device__ __forceinline__ void for_each_product(const Prob<V>& P, int a0, int a1, int lr,
+                                                 ChunkMeta<V, EC, VALS>* cm, int* s_tmp, F&& f) {
+    for_each_round<V, L, EC, PMAX, VALS>(P, a0, a1, lr, cm, s_tmp,
+                                         [&](const int(&key)[PMAX], const V(&prod)[PMAX], const bool(&ok)[PMAX]) {
+#pragma unroll
+                                             for (int j = 0; j < PMAX; ++j)
+                                                 if (ok[j]) f(key[j], prod[j]);
+                                         });
+}
+
+// Fill n ints of LDS with v using 8-byte stores where possible (p 8-byte aligned, n even).
 template <int N, int L>
 __device__ __forceinline__ void lds_fill(int* p, int v, int lr) {
-    if constexpr (N % 4 == 0 && N >= 4 * L) {
-        int4* q = reinterpret_cast<int4*>(p);
-        const int4 w = make_int4(v, v, v, v);
+    if constexpr (N % 2 == 0 && N >= 2 * L) {
+        int2* q = reinterpret_cast<int2*>(p);
+        const int2 w = make_int2(v, v);
 #pragma unroll 4
-        for (int j = lr; j < N / 4; j += L) q[j] = w;
+        for (int j = lr; j < N / 2; j += L) q[j] = w;
     } else {
         for (int j = lr; j < N; j += L) p[j] = v;
     }

@@ -34,12 +34,14 @@ NUM_GLOBAL = 10
 @pytest.fixture(scope="module")
 def emu():
     os.makedirs(BUILD, exist_ok=True)
+    tmp = f".{os.getpid()}"
     srcs = [os.path.join(CSRC, f) for f in ("k_rows.hip", "k_sym.hip", "k_symnum.hip", "api.hip")]
     cmd = ["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer", "-pthread",
            "-Wno-unused-result", "-I", os.path.join(EMU, "include"), "-I", os.path.join(ROOT, "include"),
            "-I", os.path.join(ROOT, "gpuspgemm_amd", "apps"), "-x", "c++", *srcs, "-x", "none",
-           os.path.join(EMU, "emu_runtime.cpp"), os.path.join(EMU, "emu_main.cpp"), "-o", EXE]
+           os.path.join(EMU, "emu_runtime.cpp"), os.path.join(EMU, "emu_main.cpp"), "-o", EXE + tmp]
     subprocess.run(cmd, check=True)
+    os.replace(EXE + tmp, EXE)  # parallel test workers may be running the previous build
     return EXE
 
 
