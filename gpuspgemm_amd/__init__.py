@@ -26,7 +26,7 @@ from .csr import DeviceCSR, HostCSR, _dtype_code
 
 __all__ = [
     "HostCSR", "DeviceCSR", "SpgError", "proposal", "proposal_batched", "get_nip",
-    "device_info", "set_tuning", "get_tuning", "get_variants", "profile", "load_library",
+    "device_info", "set_tuning", "get_tuning", "get_variants", "variant_list", "profile", "load_library",
 ]
 
 
@@ -97,6 +97,25 @@ def get_tuning(phase: int) -> str:
 
 def get_variants(phase: int) -> str:
     return _text(_lib.load().spg_get_variants, phase)
+
+
+def variant_list(phase: int) -> list[dict]:
+    """Compiled kernel variants of a phase as dicts: id, wg, rows, lanes
+    (per row), table (phase 0) or capacity (phase 1), chunk, slots, and the
+    flags max (phase 0's overflow-detecting variant) / global (phase 1's
+    global-memory variant)."""
+    out = []
+    for line in get_variants(phase).splitlines():
+        parts = line.split()
+        v = {"id": int(parts[0]), "max": "max" in parts[1:], "global": "global" in parts[1:]}
+        for kv in parts[1:]:
+            if "=" in kv:
+                k, x = kv.split("=")
+                v[k] = int(x)
+        if "wg" in v:
+            v["lanes"] = v["wg"] // v["rows"]
+        out.append(v)
+    return out
 
 
 class profile:

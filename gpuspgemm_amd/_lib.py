@@ -76,6 +76,11 @@ class SpgError(RuntimeError):
         super().__init__(f"{self.status_name}: {message}")
 
 
+def available() -> bool:
+    """Whether the shared library has been built (loading it needs no GPU)."""
+    return os.path.exists(LIB_PATH)
+
+
 def load() -> ctypes.CDLL:
     """Load libgpuspgemm.so (raises ImportError if it has not been built)."""
     global _lib

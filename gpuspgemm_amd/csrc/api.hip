@@ -101,24 +101,27 @@ Context g_ctx[kMaxDevices];
 // ---------------------------------------------------------------- bin tables
 
 void default_tables(BinTable& sym, BinTable& num) {
+    struct Row {
+        int variant, nip, nnz;
+    };
     // Symbolic count: bins by NIP; a row's distinct columns (<= NIP) stay
-    // below its table size, at most 3/4 full.
-    const int sym_caps[] = {8, 16, 32, 64, 128, 256, 768, 2048, 4096, INT_MAX};
+    // at most half of its table.
+    const Row srows[] = {{0, 8, INT_MAX},     {1, 16, INT_MAX},   {2, 32, INT_MAX},
+                         {3, 64, INT_MAX},    {4, 128, INT_MAX},  {5, 256, INT_MAX},
+                         {6, 768, INT_MAX},   {7, 2048, INT_MAX}, {8, 4096, INT_MAX},
+                         {kSymMaxVariant, INT_MAX, INT_MAX}};
     sym = BinTable{};
-    sym.n = static_cast<int>(sizeof(sym_caps) / sizeof(sym_caps[0]));
+    sym.n = static_cast<int>(sizeof(srows) / sizeof(srows[0]));
     for (int b = 0; b < sym.n; ++b) {
-        sym.variant[b] = b;
-        sym.rows[b] = kSymVariants[b].r;
-        sym.nip_cap[b] = sym_caps[b];
+        sym.variant[b] = srows[b].variant;
+        sym.rows[b] = kSymVariants[srows[b].variant].r;
+        sym.nip_cap[b] = srows[b].nip;
         sym.nnz_cap[b] = INT_MAX;
     }
     // Fused pass: bins by (nnz, NIP).  nnz_cap never exceeds the variant's
     // capacity; the NIP caps keep rows on variants whose lanes can hold
     // their products in registers.
-    struct Row {
-        int variant, nip, nnz;
-    };
-    const Row rows[] = {{0, 16, 8},          {1, 32, 16},        {2, 64, 32},         {3, 128, 64},
+    const Row rows[] = {{10, 16, 8},         {11, 32, 16},       {12, 64, 32},        {13, 128, 64},
                         {4, 256, 128},       {5, 768, 128},      {6, 4096, 512},      {7, INT_MAX, 1024},
                         {8, INT_MAX, 2048},  {9, INT_MAX, 8192}, {kNumGlobalVariant, INT_MAX, INT_MAX}};
     num = BinTable{};

@@ -34,54 +34,77 @@ __global__ __launch_bounds__(kRowThreads) void k_nip(ProbSet<V> ps, BinTable sym
 
     const Tile t = ps.tile(blockIdx.x);
     const Prob<V> P = ps.get(t.inst);
+    constexpr int kPer = kTileRows / kRowThreads;
+    constexpr int kShort = 4;  // entries of a row read without a loop
+    // all loads of a thread's rows are issued together: row pointers, then
+    // up to kShort columns per row, then their B row lengths
+    int a0[kPer], a1[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int r = threadIdx.x + j * kRowThreads;
+        a0[j] = a1[j] = 0;
+        if (r < t.rows) {
+            a0[j] = P.a_rpt[t.row0 + r];
+            a1[j] = P.a_rpt[t.row0 + r + 1];
+        }
+    }
+    int k[kPer][kShort];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j)
+#pragma unroll
+        for (int q = 0; q < kShort; ++q) k[j][q] = a0[j] + q < a1[j] ? P.a_col[a0[j] + q] : -1;
+    long long s[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        s[j] = 0;
+#pragma unroll
+        for (int q = 0; q < kShort; ++q)
+            if (k[j][q] >= 0) s[j] += P.b_rpt[k[j][q] + 1] - P.b_rpt[k[j][q]];
+    }
     int my_max = 0;
     long long my_total = 0;
     int my_flags = 0;
 #pragma unroll
-    for (int j = 0; j < kTileRows / kRowThreads; ++j) {
+    for (int j = 0; j < kPer; ++j) {
         const int r = threadIdx.x + j * kRowThreads;
         const bool valid = r < t.rows;
         const int row = t.row0 + r;
-        int a0 = 0, a1 = 0;
-        if (valid) {
-            a0 = P.a_rpt[row];
-            a1 = P.a_rpt[row + 1];
-        }
-        long long s = 0;
-        if (a1 - a0 <= 32) {
-            for (int p = a0; p < a1; ++p) {
-                const int k = P.a_col[p];
-                s += P.b_rpt[k + 1] - P.b_rpt[k];
+        const int len = a1[j] - a0[j];
+        if (len > kShort && len <= 32)
+            for (int p = a0[j] + kShort; p < a1[j]; ++p) {
+                const int kk = P.a_col[p];
+                s[j] += P.b_rpt[kk + 1] - P.b_rpt[kk];
             }
-        }
-        unsigned long long longs = __ballot(a1 - a0 > 32);
+        unsigned long long longs = __ballot(len > 32);
         while (longs) {
             const int src = __ffsll(static_cast<long long>(longs)) - 1;
             longs &= longs - 1;
-            const int la0 = __shfl(a0, src);
-            const int la1 = __shfl(a1, src);
+            const int la0 = __shfl(a0[j], src);
+            const int la1 = __shfl(a1[j], src);
             long long part = 0;
             for (int p = la0 + lane_id(); p < la1; p += kWave) {
-                const int k = P.a_col[p];
-                part += P.b_rpt[k + 1] - P.b_rpt[k];
+                const int kk = P.a_col[p];
+                part += P.b_rpt[kk + 1] - P.b_rpt[kk];
             }
             part = wave_sum64(part);
-            if (lane_id() == src) s = part;
+            if (lane_id() == src) s[j] = part;
         }
+        int bin = -1;
         if (valid) {
-            if (s > INT_MAX) {
+            if (s[j] > INT_MAX) {
                 my_flags |= kFlagNipOverflow;
-                s = INT_MAX;
+                s[j] = INT_MAX;
             }
-            const int nip = static_cast<int>(s);
+            const int nip = static_cast<int>(s[j]);
             P.nip[row] = nip;
             if (nip == 0)
                 P.c_rpt[row] = 0;
             else
-                atomicAdd(&s_cnt[find_bin(sym, nip, 0)], 1);
+                bin = find_bin(sym, nip, 0);
             my_max = max(my_max, nip);
-            my_total += s;
+            my_total += s[j];
         }
+        wave_count_add(s_cnt, bin);
     }
     my_max = wave_max(my_max);
     my_total = wave_sum64(my_total);
@@ -120,11 +143,9 @@ __global__ __launch_bounds__(kRowThreads) void k_scatter(ProbSet<V> ps, BinTable
         bin[j] = -1;
         if (r < t.rows) {
             const int nip = P.nip[t.row0 + r];
-            if (nip > 0) {
-                bin[j] = find_bin(sym, nip, 0);
-                rank[j] = atomicAdd(&s_cnt[bin[j]], 1);
-            }
+            if (nip > 0) bin[j] = find_bin(sym, nip, 0);
         }
+        rank[j] = wave_rank_add(s_cnt, bin[j]);
     }
     __syncthreads();
     if (threadIdx.x < sym.n && s_cnt[threadIdx.x] > 0) {
@@ -166,17 +187,18 @@ __global__ __launch_bounds__(kRowThreads) void k_scan_count(ProbSet<V> ps, BinTa
 #pragma unroll
     for (int j = 0; j < kTileRows / kRowThreads; ++j) {
         const int r = threadIdx.x + j * kRowThreads;
+        int b = -1;
         if (r < t.rows) {
             const int row = t.row0 + r;
             const int nnz = P.c_rpt[row];
             sum += nnz;
             mx = max(mx, nnz);
             if (nnz > 0) {
-                const int b = find_bin(num, P.nip[row], nnz);
-                atomicAdd(&s_cnt[b], 1);
+                b = find_bin(num, P.nip[row], nnz);
                 if (num.variant[b] == kNumGlobalVariant) pool += global_table_slots(nnz);
             }
         }
+        wave_count_add(s_cnt, b);
     }
     sum = wave_sum64(sum);
     mx = wave_max(mx);
@@ -246,11 +268,8 @@ __global__ __launch_bounds__(kRowThreads) void k_scan_apply(ProbSet<V> ps, BinTa
         const int r = r0 + j;
         v[j] = r < t.rows ? P.c_rpt[t.row0 + r] : 0;
         local += v[j];
-        bin[j] = -1;
-        if (v[j] > 0) {
-            bin[j] = find_bin(num, P.nip[t.row0 + r], v[j]);
-            rank[j] = atomicAdd(&s_cnt[bin[j]], 1);
-        }
+        bin[j] = v[j] > 0 ? find_bin(num, P.nip[t.row0 + r], v[j]) : -1;
+        rank[j] = wave_rank_add(s_cnt, bin[j]);
     }
     // block exclusive scan of `local`
     long long incl = local;

@@ -18,46 +18,59 @@
 
 namespace spg {
 
-// Compact the occupied slots of a row's table into cols[0..n) (unsorted);
-// returns the number of occupied slots.
+// Compact the occupied slots of a row's table into cols[0..min(n, P))
+// (unsorted); returns the number n of occupied slots.  Every lane reads its
+// T/L slots first, then one group scan of the per-lane counts places them.
 template <int L, int T, int P>
 __device__ __forceinline__ int compact_table(const int* tab, int* cols, int lr, int* s_tmp) {
     static_assert(T % L == 0, "table must be a multiple of the group");
+    constexpr int S = T / L;
+    constexpr int B = L > kWave && S > 8 ? 8 : S;  // slots read per batch (workgroup-wide rows: bounded registers)
     int n = 0;
-    for (int j0 = 0; j0 < T; j0 += L) {
-        const int key = tab[j0 + lr];
-        const bool has = key != kEmpty;
-        if constexpr (L == 1) {
-            if (has) {
-                if (n < P) cols[n] = key;
-                ++n;
-            }
-        } else if constexpr (L <= kWave) {
-            const unsigned long long m = __ballot(has);
-            const int shift = (threadIdx.x % kWave) / L * L;
-            const unsigned long long seg = L == kWave ? m : (m >> shift) & ((1ull << L) - 1);
-            const int pos = n + __popcll(seg & ((1ull << lr) - 1));
-            if (has && pos < P) cols[pos] = key;
-            n += __popcll(seg);
-        } else {
-            int tot;
-            const int ex = group_excl_scan<L>(has ? 1 : 0, lr, s_tmp, tot);
-            if (has && n + ex < P) cols[n + ex] = key;
-            n += tot;
+    for (int b0 = 0; b0 < S; b0 += B) {
+        int key[B];
+        int mine = 0;
+#pragma unroll
+        for (int s = 0; s < B; ++s) {
+            key[s] = tab[lr + (b0 + s) * L];
+            mine += key[s] != kEmpty;
         }
+        int tot;
+        int pos = n + group_excl_scan<L>(mine, lr, s_tmp, tot);
+#pragma unroll
+        for (int s = 0; s < B; ++s) {
+            if (key[s] == kEmpty) continue;
+            if (pos < P) cols[pos] = key[s];
+            ++pos;
+        }
+        n += tot;
     }
     return n;
 }
 
-// Ascending list of the occupied slots of a row's table in cols[0..n),
-// padded with INT_MAX up to P; returns the number of occupied slots.
+// Sorted column list of a row: the occupied slots of its table in
+// ascending order in cols[0..n), padded with INT_MAX up to P; returns n.
+// Groups inside one wave sort in registers (bitonic_sort_regs) and also
+// leave lane lr's share cols[lr*E .. lr*E + E) in x; larger groups run the
+// bitonic network in LDS.
 template <int L, int T, int P>
-__device__ __forceinline__ int sort_row(const int* tab, int* cols, int lr, int* s_tmp) {
+__device__ __forceinline__ int sort_row(const int* tab, int* cols, int lr, int* s_tmp, int (&x)[L <= kWave ? P / L : 1]) {
     const int found = compact_table<L, T, P>(tab, cols, lr, s_tmp);
     const int n = min(found, P);
-    for (int j = n + lr; j < P; j += L) cols[j] = INT_MAX;
-    gsync<L>();
-    bitonic_sort<P, L>(cols, lr);
+    if constexpr (L <= kWave) {
+        constexpr int E = P / L;
+        gsync<L>();
+#pragma unroll
+        for (int s = 0; s < E; ++s) x[s] = lr * E + s < n ? cols[lr * E + s] : INT_MAX;
+        bitonic_sort_regs<P, L>(x, lr);
+        gsync<L>();
+#pragma unroll
+        for (int s = 0; s < E; ++s) cols[lr * E + s] = x[s];
+    } else {
+        for (int j = n + lr; j < P; j += L) cols[j] = INT_MAX;
+        gsync<L>();
+        bitonic_sort<P, L>(cols, lr);
+    }
     return found;
 }
 
@@ -87,11 +100,13 @@ __device__ __forceinline__ void symnum_rows(const ProbSet<V>& ps, const int2* __
     const bool active = idx < end;
     const int2 it = active ? items[idx] : make_int2(0, 0);
     const Prob<V> Pr = ps.get(it.x);
-    int a0 = 0, a1 = 0, nip = 0;
+    int a0 = 0, a1 = 0, nip = 0, c0 = 0, cn = 0;
     if (active) {
         a0 = Pr.a_rpt[it.y];
         a1 = Pr.a_rpt[it.y + 1];
         nip = Pr.nip[it.y];
+        c0 = Pr.c_rpt[it.y];
+        cn = Pr.c_rpt[it.y + 1] - c0;
     }
     // products staged in registers when the whole row fits in one round
     const bool fits = a1 - a0 <= EC && nip <= L * PMAX;
@@ -116,34 +131,45 @@ __device__ __forceinline__ void symnum_rows(const ProbSet<V>& ps, const int2* __
     }
     gsync<L>();
 
-    const int found = sort_row<L, T, P>(tab, cols, lr, s_tmp);
+    constexpr int E = L <= kWave ? P / L : 1;
+    int x[E];
+    const int found = sort_row<L, T, P>(tab, cols, lr, s_tmp, x);
     const int n = min(found, P);
 
-    int c0 = 0, w = 0;
+    int w = 0;
     if (active) {
-        c0 = Pr.c_rpt[it.y];
-        const int cn = Pr.c_rpt[it.y + 1] - c0;
         if (lr == 0 && cn != found) atomicOr(&meta->flags, kFlagMismatch);
         w = min(n, cn);
     }
-    for (int j = lr; j < w; j += L) Pr.c_col[c0 + j] = cols[j];
+    if constexpr (L <= kWave) {
+#pragma unroll
+        for (int s = 0; s < E; ++s)
+            if (lr * E + s < w) Pr.c_col[c0 + lr * E + s] = x[s];
+    } else {
+        for (int j = lr; j < w; j += L) Pr.c_col[c0 + j] = cols[j];
+    }
     for (int j = lr; j < P; j += L) vals[j] = V(0);
     gsync<L>();
 
-    auto accumulate = [&](int k, V v) {
-        const int pos = lower_bound_pow2<P>(cols, k);
-        if constexpr (L == 1)
-            vals[pos] += v;
-        else
-            lds_add(vals + pos, v);
-    };
-    if (fits) {
+    // binary search of every product's column in the sorted list
+    // (find_key, num.cuh:28-41), all of a lane's products in lockstep, then
+    // LDS atomic adds that nothing waits for
+    // (in groups of up to 8 keys, which bounds the registers in flight)
+    auto accumulate = [&](const int(&k)[PMAX], const V(&v)[PMAX], const bool(&o)[PMAX]) {
+        constexpr int G = PMAX % 8 == 0 ? 8 : (PMAX % 6 == 0 ? 6 : (PMAX % 4 == 0 ? 4 : PMAX));
+        static_for<0, PMAX / G>([&](auto gi) {
+            constexpr int J0 = decltype(gi)::value * G;
+            int pos[G];
+            lower_bound_all<P, G, J0>(cols, k, pos);
 #pragma unroll
-        for (int j = 0; j < PMAX; ++j)
-            if (ok[j]) accumulate(key[j], prod[j]);
-    } else {
-        for_each_product<V, L, EC, PMAX, true>(Pr, a0, a1, lr, cm, s_tmp, accumulate);
-    }
+            for (int j = 0; j < G; ++j)
+                if (o[J0 + j]) lds_add(vals + pos[j], v[J0 + j]);
+        });
+    };
+    if (fits)
+        accumulate(key, prod, ok);
+    else
+        for_each_round<V, L, EC, PMAX, true>(Pr, a0, a1, lr, cm, s_tmp, accumulate);
     gsync<L>();
     for (int j = lr; j < w; j += L) Pr.c_val[c0 + j] = vals[j];
     gsync<L>();  // LDS is reused by the next unit
@@ -296,7 +322,7 @@ static int occupancy_variant() {
     return n > 0 ? n : 1;
 }
 
-#define SPG_NUM_VARIANTS(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9)
+#define SPG_NUM_VARIANTS(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13)
 
 template <typename V>
 hipError_t launch_symnum_bin(int variant, const ProbSet<V>& ps, const int2* items, int item_off, int count,

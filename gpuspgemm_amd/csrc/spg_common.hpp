@@ -21,6 +21,7 @@
 #include <climits>
 #include <cstddef>
 #include <cstdint>
+#include <type_traits>
 
 namespace spg {
 
@@ -136,6 +137,17 @@ struct MultiArgs {
 
 // ---------------------------------------------------------------- device helpers
 
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [B, E).
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+
+
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
 template <int T>
@@ -217,6 +229,37 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
+// Wave-aggregated LDS counters: every lane of the wave calls with its bin
+// (negative: none); one atomic per distinct bin of the wave instead of one
+// per lane on the same address.  wave_rank_add also returns each lane's
+// rank among the lanes counted before it.
+__device__ __forceinline__ void wave_count_add(int* cnt, int bin) {
+    unsigned long long todo = __ballot(bin >= 0);
+    while (todo) {
+        const int leader = __ffsll(static_cast<long long>(todo)) - 1;
+        const int b = __shfl(bin, leader);
+        const unsigned long long m = __ballot(bin == b);
+        if (lane_id() == leader) atomicAdd(&cnt[b], __popcll(m));
+        todo &= ~m;
+    }
+}
+
+__device__ __forceinline__ int wave_rank_add(int* cnt, int bin) {
+    int rank = 0;
+    unsigned long long todo = __ballot(bin >= 0);
+    while (todo) {
+        const int leader = __ffsll(static_cast<long long>(todo)) - 1;
+        const int b = __shfl(bin, leader);
+        const unsigned long long m = __ballot(bin == b);
+        int base = 0;
+        if (lane_id() == leader) base = atomicAdd(&cnt[b], __popcll(m));
+        base = __shfl(base, leader);
+        if (bin == b) rank = base + __popcll(m & ((1ull << lane_id()) - 1));
+        todo &= ~m;
+    }
+    return rank;
+}
+
 // Branch-free lower bound in a sorted LDS list of P entries (P a power of
 // two, padded with INT_MAX).  The key is known to be present.
 template <int P>
@@ -226,6 +269,24 @@ __device__ __forceinline__ int lower_bound_pow2(const int* cols, int key) {
     for (int step = P >> 1; step > 0; step >>= 1)
         if (cols[pos + step - 1] < key) pos += step;
     return pos;
+}
+
+// Lower bounds of keys[J0, J0 + N), searched in lockstep: each halving step
+// reads the probes of all N keys at once, so a lane waits log2(P) LDS round
+// trips for the N keys instead of N * log2(P).
+template <int P, int N, int J0 = 0, int K>
+__device__ __forceinline__ void lower_bound_all(const int* cols, const int (&key)[K], int (&pos)[N]) {
+    static_assert(J0 + N <= K, "key range out of bounds");
+#pragma unroll
+    for (int j = 0; j < N; ++j) pos[j] = 0;
+#pragma unroll
+    for (int step = P >> 1; step > 0; step >>= 1) {
+        int probe[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) probe[j] = cols[pos[j] + step - 1];
+#pragma unroll
+        for (int j = 0; j < N; ++j) pos[j] += probe[j] < key[J0 + j] ? step : 0;
+    }
 }
 
 template <typename V>

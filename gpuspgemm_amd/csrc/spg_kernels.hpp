@@ -32,17 +32,19 @@ constexpr int kSymMaxVariant = 9;
 constexpr int kSymMaxThreshold = 26214;  // distinct keys admitted in the 32768-slot table (80 %)
 constexpr int kSymMultiLds = 36 * 1024;  // bytes: max over variants 0..8 of r * sym_row_ints * 4
 
-// Fused sorted-columns + numeric variants, one kernel each: 0..8 use 256
-// threads, 9 a 1024-thread workgroup per row; rows above its capacity use
-// global-memory tables (variant 10).  A row whose products fit in one
-// chunk and in lanes * pmax keeps them in registers between the two halves.
+// Fused sorted-columns + numeric variants, one kernel each: all use 256
+// threads except 9 (a 1024-thread workgroup per row); rows above every
+// capacity use global-memory tables (variant 14).  A row whose products fit
+// in one chunk and in lanes * pmax keeps them in registers between the two
+// halves.  Variants 10..13 are the small capacities with more lanes per row.
 constexpr NumVariant kNumVariants[] = {
     {256, 256, 8, 4, 16},   {256, 128, 16, 4, 16}, {256, 64, 32, 8, 16},  {256, 32, 64, 16, 16},
     {256, 16, 128, 32, 16}, {256, 4, 128, 64, 12}, {256, 4, 512, 64, 16}, {256, 1, 1024, 256, 8},
-    {256, 1, 2048, 256, 8}, {1024, 1, 8192, 1024, 8},
+    {256, 1, 2048, 256, 8}, {1024, 1, 8192, 1024, 4}, {256, 32, 8, 8, 2}, {256, 16, 16, 16, 2},
+    {256, 16, 32, 16, 4},   {256, 8, 64, 32, 4},
 };
-constexpr int kNumLdsVariants = 10;  // variants with an LDS kernel (0..9)
-constexpr int kNumGlobalVariant = 10;
+constexpr int kNumLdsVariants = 14;  // variants with an LDS kernel (0..13)
+constexpr int kNumGlobalVariant = 14;
 
 constexpr int kGlobalSortChunk = 4096;  // LDS chunk of the global-bin column sort
 
@@ -72,7 +74,15 @@ constexpr int max_sym_multi_lds() {
     }
     return m;
 }
-constexpr int kNumBigLds = 4 * kNumVariants[kNumLdsVariants - 1].r * num_row_ints(kNumVariants[kNumLdsVariants - 1]);
+constexpr int max_num_lds() {
+    int m = 0;
+    for (int i = 0; i < kNumLdsVariants; ++i) {
+        const int b = 4 * kNumVariants[i].r * num_row_ints(kNumVariants[i]);
+        m = m > b ? m : b;
+    }
+    return m;
+}
+constexpr int kNumBigLds = max_num_lds();
 static_assert(max_sym_multi_lds() <= kSymMultiLds, "symbolic multi-bin LDS budget");
 
 // Slots of a global-memory hash table for a row with `keys` distinct keys.

@@ -279,4 +279,43 @@ __device__ __forceinline__ void bitonic_sort(int* cols, int lr) {
     }
 }
 
+// Ascending bitonic sort of P ints held in registers by the L lanes of a
+// row group (L <= 64, P a power of two, P >= L): lane lr holds elements
+// lr * E .. lr * E + E - 1 in x[0..E).  Stages whose partner lies in the
+// same lane are register compare-exchanges; the others exchange with lane
+// lr ^ (j / E) through cross-lane shuffles, E of them in flight at once.
+template <int P, int L>
+__device__ __forceinline__ void bitonic_sort_regs(int (&x)[P / L], int lr) {
+    constexpr int E = P / L;
+    static_assert(E >= 1 && E * L == P && L <= kWave, "bad register sort shape");
+#pragma unroll
+    for (int k = 2; k <= P; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= E) {
+                const int lj = j / E;
+                const bool lower = (lr & lj) == 0;
+                int other[E];
+#pragma unroll
+                for (int s = 0; s < E; ++s) other[s] = __shfl_xor(x[s], lj, L);
+#pragma unroll
+                for (int s = 0; s < E; ++s) {
+                    const bool asc = ((lr * E + s) & k) == 0;
+                    x[s] = (lower == asc) ? min(x[s], other[s]) : max(x[s], other[s]);
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < E; ++s) {
+                    if (s & j) continue;
+                    const bool asc = ((lr * E + s) & k) == 0;
+                    const int lo = min(x[s], x[s + j]);
+                    const int hi = max(x[s], x[s + j]);
+                    x[s] = asc ? lo : hi;
+                    x[s + j] = asc ? hi : lo;
+                }
+            }
+        }
+    }
+}
+
 }  // namespace spg

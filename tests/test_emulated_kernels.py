@@ -21,14 +21,17 @@ EXE = os.path.join(BUILD, "emu_spgemm")
 CSRC = os.path.join(ROOT, "gpuspgemm_amd", "csrc")
 INT_MAX = 2**31 - 1
 
-# (lanes per row, A entries per chunk, table slots) of the symbolic variants
-SYM = [(1, 4, 16), (2, 4, 32), (4, 8, 64), (8, 16, 128), (16, 32, 256), (64, 64, 512), (64, 64, 1024),
-       (256, 256, 4096), (256, 256, 8192)]
-SYM_MAX = 9
-# (lanes per row, A entries per chunk, capacity) of the fused variants
-NUM = [(1, 4, 8), (2, 4, 16), (4, 8, 32), (8, 16, 64), (16, 32, 128), (64, 64, 128), (64, 64, 512),
-       (256, 256, 1024), (256, 256, 2048), (1024, 1024, 8192)]
-NUM_GLOBAL = 10
+import gpuspgemm_amd as g
+
+# Compiled variants, read from the library (no GPU needed): symbolic
+# (lanes per row, A entries per chunk, table slots) and fused (lanes, chunk,
+# capacity).
+_SV = g.variant_list(0)
+_NV = g.variant_list(1)
+SYM = {v["id"]: (v["lanes"], v["chunk"], v["table"]) for v in _SV if not v["max"]}
+SYM_MAX = next(v["id"] for v in _SV if v["max"])
+NUM = {v["id"]: (v["lanes"], v["chunk"], v["capacity"]) for v in _NV if not v["global"]}
+NUM_GLOBAL = next(v["id"] for v in _NV if v["global"])
 
 
 @pytest.fixture(scope="module")
@@ -91,7 +94,7 @@ def test_rectangular_and_empty_rows(emu, tmp_path):
     run_emu(emu, tmp_path, A, B)
 
 
-@pytest.mark.parametrize("variant", range(len(SYM)))
+@pytest.mark.parametrize("variant", sorted(SYM))
 def test_symbolic_variant(emu, tmp_path, variant):
     lanes, ec, t = SYM[variant]
     cap = t // 2
@@ -102,7 +105,7 @@ def test_symbolic_variant(emu, tmp_path, variant):
     run_emu(emu, tmp_path, *chunked(m, min(cap, 2 * ec + 1), seed=variant), tables=table)
 
 
-@pytest.mark.parametrize("variant", range(len(NUM)))
+@pytest.mark.parametrize("variant", sorted(NUM))
 def test_fused_variant(emu, tmp_path, variant):
     lanes, ec, p = NUM[variant]
     table = {1: f"{variant} {INT_MAX} {p}\n{NUM_GLOBAL} {INT_MAX} {INT_MAX}\n"}

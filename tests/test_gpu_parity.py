@@ -165,26 +165,38 @@ def _mixed_case():
     return A, A
 
 
-SYM_VARIANT_T = [16, 32, 64, 128, 256, 512, 1024, 4096, 8192]   # multi-bin symbolic variants
-SYM_MAX_VARIANT = 9
-NUM_VARIANT_P = [8, 16, 32, 64, 128, 128, 512, 1024, 2048, 8192]  # fused LDS variants
-NUM_GLOBAL_VARIANT = 10
+def _variants(phase):
+    return g.variant_list(phase)
 
 
-@pytest.mark.parametrize("variant", range(len(SYM_VARIANT_T)))
+SYM_MULTI = [v["id"] for v in _variants(0) if not v["max"]] if g._lib.available() else list(range(13))
+NUM_LDS = [v["id"] for v in _variants(1) if not v["global"]] if g._lib.available() else list(range(14))
+
+
+def _sym_max():
+    return next(v["id"] for v in _variants(0) if v["max"])
+
+
+def _num_global():
+    return next(v["id"] for v in _variants(1) if v["global"])
+
+
+@pytest.mark.parametrize("variant", SYM_MULTI)
 def test_every_symbolic_variant(gpu_lib, variant):
     A, B = _mixed_case()
-    g.set_tuning(0, f"{variant} {SYM_VARIANT_T[variant] // 2} {INT_MAX}\n{SYM_MAX_VARIANT} {INT_MAX} {INT_MAX}\n")
+    t = next(v["table"] for v in _variants(0) if v["id"] == variant)
+    g.set_tuning(0, f"{variant} {t // 2} {INT_MAX}\n{_sym_max()} {INT_MAX} {INT_MAX}\n")
     try:
         compare(A, B, gpu_multiply(A, B))
     finally:
         g.set_tuning(0, None)
 
 
-@pytest.mark.parametrize("variant", range(len(NUM_VARIANT_P)))
+@pytest.mark.parametrize("variant", NUM_LDS)
 def test_every_fused_variant(gpu_lib, variant):
     A, B = _mixed_case()
-    g.set_tuning(1, f"{variant} {INT_MAX} {NUM_VARIANT_P[variant]}\n{NUM_GLOBAL_VARIANT} {INT_MAX} {INT_MAX}\n")
+    p = next(v["capacity"] for v in _variants(1) if v["id"] == variant)
+    g.set_tuning(1, f"{variant} {INT_MAX} {p}\n{_num_global()} {INT_MAX} {INT_MAX}\n")
     try:
         compare(A, B, gpu_multiply(A, B))
     finally:
@@ -195,8 +207,8 @@ def test_global_paths_forced(gpu_lib):
     """Every row through the largest-table and global kernels."""
     A = gen.random_csr(600, 800, 0.02, seed=31)
     B = gen.random_csr(800, 700, 0.02, seed=32)
-    g.set_tuning(0, f"{SYM_MAX_VARIANT} {INT_MAX} {INT_MAX}\n")
-    g.set_tuning(1, f"{NUM_GLOBAL_VARIANT} {INT_MAX} {INT_MAX}\n")
+    g.set_tuning(0, f"{_sym_max()} {INT_MAX} {INT_MAX}\n")
+    g.set_tuning(1, f"{_num_global()} {INT_MAX} {INT_MAX}\n")
     try:
         compare(A, B, gpu_multiply(A, B))
     finally:
