@@ -100,38 +100,31 @@ Context g_ctx[kMaxDevices];
 
 // ---------------------------------------------------------------- bin tables
 
+struct TunedBin {
+    int variant, nip, nnz;
+};
+// kTunedSym / kTunedNum: the default bin tables, written by the tuner
+// (gpuspgemm_amd/tune.py) from measurements on the target GPU.
+#include "tuned_tables.inc"
+
+template <size_t N>
+void fill_table(BinTable& t, const TunedBin (&rows)[N], bool sym_phase) {
+    t = BinTable{};
+    t.n = static_cast<int>(N);
+    for (int b = 0; b < t.n; ++b) {
+        const int v = rows[b].variant;
+        t.variant[b] = v;
+        t.rows[b] = sym_phase ? kSymVariants[v].r : (v < kNumGlobalVariant ? kNumVariants[v].r : 1);
+        t.nip_cap[b] = rows[b].nip;
+        t.nnz_cap[b] = rows[b].nnz;
+    }
+}
+
 void default_tables(BinTable& sym, BinTable& num) {
-    struct Row {
-        int variant, nip, nnz;
-    };
-    // Symbolic count: bins by NIP; a row's distinct columns (<= NIP) stay
-    // at most half of its table.
-    const Row srows[] = {{0, 8, INT_MAX},     {1, 16, INT_MAX},   {2, 32, INT_MAX},
-                         {3, 64, INT_MAX},    {4, 128, INT_MAX},  {5, 256, INT_MAX},
-                         {6, 768, INT_MAX},   {7, 2048, INT_MAX}, {8, 4096, INT_MAX},
-                         {kSymMaxVariant, INT_MAX, INT_MAX}};
-    sym = BinTable{};
-    sym.n = static_cast<int>(sizeof(srows) / sizeof(srows[0]));
-    for (int b = 0; b < sym.n; ++b) {
-        sym.variant[b] = srows[b].variant;
-        sym.rows[b] = kSymVariants[srows[b].variant].r;
-        sym.nip_cap[b] = srows[b].nip;
-        sym.nnz_cap[b] = INT_MAX;
-    }
-    // Fused pass: bins by (nnz, NIP).  nnz_cap never exceeds the variant's
-    // capacity; the NIP caps keep rows on variants whose lanes can hold
-    // their products in registers.
-    const Row rows[] = {{10, 16, 8},         {11, 32, 16},       {12, 64, 32},        {13, 128, 64},
-                        {4, 256, 128},       {5, 768, 128},      {6, 4096, 512},      {7, INT_MAX, 1024},
-                        {8, INT_MAX, 2048},  {9, INT_MAX, 8192}, {kNumGlobalVariant, INT_MAX, INT_MAX}};
-    num = BinTable{};
-    num.n = static_cast<int>(sizeof(rows) / sizeof(rows[0]));
-    for (int b = 0; b < num.n; ++b) {
-        num.variant[b] = rows[b].variant;
-        num.rows[b] = rows[b].variant < kNumGlobalVariant ? kNumVariants[rows[b].variant].r : 1;
-        num.nip_cap[b] = rows[b].nip;
-        num.nnz_cap[b] = rows[b].nnz;
-    }
+    static_assert(sizeof(kTunedSym) / sizeof(kTunedSym[0]) <= kMaxBins, "too many symbolic bins");
+    static_assert(sizeof(kTunedNum) / sizeof(kTunedNum[0]) <= kMaxBins, "too many fused bins");
+    fill_table(sym, kTunedSym, true);
+    fill_table(num, kTunedNum, false);
 }
 
 // A symbolic table is a run of multi-bin variants followed by the max
@@ -197,6 +190,8 @@ spg_status ensure_context(Context*& out) {
     }
     SPG_HIP(hipHostMalloc(reinterpret_cast<void**>(&c.h_meta), sizeof(DevMeta), hipHostMallocDefault));
     default_tables(c.sym, c.num);
+    SPG_TRY(check_sym_table(c.sym));  // a stale tuned_tables.inc fails here, loudly
+    SPG_TRY(check_num_table(c.num));
     c.occ_sym_multi = occupancy_sym_multi<double>();
     c.occ_sym_max = occupancy_sym_max<double>();
     for (int v = 0; v < kNumLdsVariants; ++v) {
@@ -463,6 +458,17 @@ spg_status multiply_impl(Context& c, int count, const spg_csr* A, const spg_csr*
         prof.begin("scatter_sym", total_rows, n_tiles);
         launch_scatter<V>(ps, c.sym, meta, items1, s);
         prof.end();
+        // The largest-table bin runs on an aux stream beside the multi-bin
+        // launch (it is queued first: its workgroups without a row exit at
+        // once and leave the CUs to the multi-bin grid).
+        SPG_HIP(hipEventRecord(c.ev_ready, s));
+        SPG_HIP(hipStreamWaitEvent(c.aux[0], c.ev_ready, 0));
+        {
+            const int grid = c.cus * c.occ_sym_max;
+            prof.begin("sym_max", 0, grid, c.aux[0]);
+            launch_sym_max<V>(ps, items1, c.sym, c.sym.n - 1, meta, fail_items, fail_off, grid, c.aux[0]);
+            prof.end();
+        }
         const int n_multi = c.sym.n - 1;
         if (n_multi > 0) {
             int grid = c.cus * c.occ_sym_multi;
@@ -471,12 +477,8 @@ spg_status multiply_impl(Context& c, int count, const spg_csr* A, const spg_csr*
             launch_sym_multi<V>(ps, items1, c.sym, n_multi, meta, grid, s);
             prof.end();
         }
-        {
-            const int grid = c.cus * c.occ_sym_max;
-            prof.begin("sym_max", 0, grid);
-            launch_sym_max<V>(ps, items1, c.sym, c.sym.n - 1, meta, fail_items, fail_off, grid, s);
-            prof.end();
-        }
+        SPG_HIP(hipEventRecord(c.ev_done[0], c.aux[0]));
+        SPG_HIP(hipStreamWaitEvent(s, c.ev_done[0], 0));
         return SPG_OK;
     };
     auto run_scan = [&](int after_fail) -> spg_status {
@@ -497,11 +499,11 @@ spg_status multiply_impl(Context& c, int count, const spg_csr* A, const spg_csr*
     spg_status st = run_front();
     if (st == SPG_OK) st = run_scan(0);
     void* sym_pool = nullptr;
-    if (st == SPG_OK && c.h_meta->fail_count > 0) {
+    if (st == SPG_OK && c.h_meta->fail_count.v > 0) {
         // rows that overflowed the largest LDS table: count them again in
         // global memory, then redo the scan stage
-        const int fails = c.h_meta->fail_count;
-        st = dev_alloc(c, &sym_pool, sizeof(int32_t) * c.h_meta->fail_pool);
+        const int fails = c.h_meta->fail_count.v;
+        st = dev_alloc(c, &sym_pool, sizeof(int32_t) * c.h_meta->fail_pool.v);
         if (st == SPG_OK) {
             prof.begin("sym_global", fails, fails);
             launch_sym_global<V>(ps, fail_items, fail_off, static_cast<int*>(sym_pool), fails, s);
@@ -516,11 +518,11 @@ spg_status multiply_impl(Context& c, int count, const spg_csr* A, const spg_csr*
         return st;
     }
     const DevMeta hm = *c.h_meta;
-    if (hm.flags & kFlagNipOverflow) {
+    if (hm.flags.v & kFlagNipOverflow) {
         cleanup();
         return fail(SPG_ERR_OVERFLOW, "a row has 2^31 or more intermediate products");
     }
-    if (hm.flags & kFlagNnzOverflow) {
+    if (hm.flags.v & kFlagNnzOverflow) {
         cleanup();
         return fail(SPG_ERR_OVERFLOW, "nnz(C) does not fit int32");
     }
@@ -528,7 +530,7 @@ spg_status multiply_impl(Context& c, int count, const spg_csr* A, const spg_csr*
     // exact-size outputs
     for (int i = 0; i < count; ++i) {
         const long long nnz = batch ? reinterpret_cast<long long*>(c.h_stage + stage_inst)[i]
-                                    : (A[i].m > 0 ? hm.nnz_total : 0);
+                                    : (A[i].m > 0 ? hm.nnz_total.v : 0);
         if (A[i].m == 0) continue;
         if (nnz >= INT_MAX) {
             cleanup();
@@ -570,7 +572,7 @@ spg_status multiply_impl(Context& c, int count, const spg_csr* A, const spg_csr*
     {
         int item_off = 0;
         for (int b = 0; b < c.num.n; ++b) {
-            const int cnt = hm.num_count[b];
+            const int cnt = hm.num_count[b].v;
             const int v = c.num.variant[b];
             if (cnt > 0) {
                 if (v == kNumGlobalVariant) {
@@ -592,7 +594,7 @@ spg_status multiply_impl(Context& c, int count, const spg_csr* A, const spg_csr*
     auto stream_of = [&](int k) { return k % Context::kStreams == 0 ? s : c.aux[k % Context::kStreams - 1]; };
     void* num_pool = nullptr;
     if (glob_bin >= 0) {
-        st = dev_alloc(c, &num_pool, sizeof(int32_t) * hm.gnum_pool);
+        st = dev_alloc(c, &num_pool, sizeof(int32_t) * hm.gnum_pool.v);
         if (st != SPG_OK) {
             cleanup();
             return st;
@@ -606,8 +608,8 @@ spg_status multiply_impl(Context& c, int count, const spg_csr* A, const spg_csr*
     int k = 0;
     if (e == hipSuccess && glob_bin >= 0) {
         const hipStream_t on = stream_of(k++);
-        prof.begin("symnum_global", hm.num_count[glob_bin], hm.num_count[glob_bin], on);
-        launch_symnum_global<V>(ps, items2, gpool_off, glob_off, hm.num_count[glob_bin],
+        prof.begin("symnum_global", hm.num_count[glob_bin].v, hm.num_count[glob_bin].v, on);
+        launch_symnum_global<V>(ps, items2, gpool_off, glob_off, hm.num_count[glob_bin].v,
                                 static_cast<int*>(num_pool), meta, on);
         prof.end();
     }
@@ -628,14 +630,14 @@ spg_status multiply_impl(Context& c, int count, const spg_csr* A, const spg_csr*
     }
     if (e == hipSuccess) e = hipGetLastError();
     if (e == hipSuccess)
-        e = hipMemcpyAsync(&c.h_meta->flags, &meta->flags, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+        e = hipMemcpyAsync(&c.h_meta->flags.v, &meta->flags.v, sizeof(int32_t), hipMemcpyDeviceToHost, s);
     if (num_pool) (void)dev_release(c, num_pool);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
         cleanup();
         return hip_fail(e, "fused pass", __FILE__, __LINE__);
     }
-    if (c.h_meta->flags & kFlagMismatch) {
+    if (c.h_meta->flags.v & kFlagMismatch) {
         cleanup();
         return fail(SPG_ERR_INTERNAL, "fused pass found a different column count than the count pass "
                                       "(are B's rows free of duplicate columns and indices in range?)");

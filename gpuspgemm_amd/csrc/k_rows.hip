@@ -8,10 +8,10 @@
 
 namespace spg {
 
-__global__ __launch_bounds__(64) void k_clear(DevMeta* meta, int from_word) {
+__global__ __launch_bounds__(256) void k_clear(DevMeta* meta, int from_word) {
     int* w = reinterpret_cast<int*>(meta);
     constexpr int kWords = sizeof(DevMeta) / sizeof(int);
-    for (int i = from_word + threadIdx.x; i < kWords; i += 64) w[i] = 0;
+    for (int i = from_word + threadIdx.x; i < kWords; i += 256) w[i] = 0;
 }
 
 // NIP_i = sum over k in A(i,:) of nnz(B(k,:)) (setup.cuh:48-91): a thread
@@ -70,10 +70,14 @@ __global__ __launch_bounds__(kRowThreads) void k_nip(ProbSet<V> ps, BinTable sym
         const bool valid = r < t.rows;
         const int row = t.row0 + r;
         const int len = a1[j] - a0[j];
-        if (len > kShort && len <= 32)
-            for (int p = a0[j] + kShort; p < a1[j]; ++p) {
-                const int kk = P.a_col[p];
-                s[j] += P.b_rpt[kk + 1] - P.b_rpt[kk];
+        if (len > kShort && len <= 32)  // rest of the row, 8 entries' loads in flight at a time
+            for (int base = a0[j] + kShort; base < a1[j]; base += 8) {
+                int kk[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) kk[q] = base + q < a1[j] ? P.a_col[base + q] : -1;
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (kk[q] >= 0) s[j] += P.b_rpt[kk[q] + 1] - P.b_rpt[kk[q]];
             }
         unsigned long long longs = __ballot(len > 32);
         while (longs) {
@@ -115,11 +119,11 @@ __global__ __launch_bounds__(kRowThreads) void k_nip(ProbSet<V> ps, BinTable sym
     if (my_flags) atomicOr(&s_flags, my_flags);
     __syncthreads();
     if (threadIdx.x < sym.n && s_cnt[threadIdx.x] > 0)
-        atomicAdd(&meta->sym_count[threadIdx.x], s_cnt[threadIdx.x]);
+        atomicAdd(&meta->sym_count[threadIdx.x].v, s_cnt[threadIdx.x]);
     if (threadIdx.x == 0) {
-        atomicMax(&meta->max_nip, s_max);
-        atomicAdd(&meta->total_nip, s_total);
-        if (s_flags) atomicOr(&meta->flags, s_flags);
+        atomicMax(&meta->max_nip.v, s_max);
+        atomicAdd(&meta->total_nip.v, s_total);
+        if (s_flags) atomicOr(&meta->flags.v, s_flags);
     }
 }
 
@@ -150,8 +154,8 @@ __global__ __launch_bounds__(kRowThreads) void k_scatter(ProbSet<V> ps, BinTable
     __syncthreads();
     if (threadIdx.x < sym.n && s_cnt[threadIdx.x] > 0) {
         int off = 0;
-        for (int b = 0; b < static_cast<int>(threadIdx.x); ++b) off += meta->sym_count[b];
-        s_base[threadIdx.x] = off + atomicAdd(&meta->sym_cursor[threadIdx.x], s_cnt[threadIdx.x]);
+        for (int b = 0; b < static_cast<int>(threadIdx.x); ++b) off += meta->sym_count[b].v;
+        s_base[threadIdx.x] = off + atomicAdd(&meta->sym_cursor[threadIdx.x].v, s_cnt[threadIdx.x]);
     }
     __syncthreads();
 #pragma unroll
@@ -167,7 +171,7 @@ __global__ __launch_bounds__(kRowThreads) void k_scan_count(ProbSet<V> ps, BinTa
                                                             long long* __restrict__ tile_sums, int after_fail) {
     // Rows that overflowed the largest LDS table have no count yet: leave
     // c_rpt untouched; the host recounts them and runs the scan again.
-    if (!after_fail && meta->fail_count > 0) return;
+    if (!after_fail && meta->fail_count.v > 0) return;
     __shared__ int s_cnt[kMaxBins];
     __shared__ long long s_sum;
     __shared__ unsigned long long s_pool;
@@ -210,11 +214,11 @@ __global__ __launch_bounds__(kRowThreads) void k_scan_count(ProbSet<V> ps, BinTa
     }
     __syncthreads();
     if (threadIdx.x < num.n && s_cnt[threadIdx.x] > 0)
-        atomicAdd(&meta->num_count[threadIdx.x], s_cnt[threadIdx.x]);
+        atomicAdd(&meta->num_count[threadIdx.x].v, s_cnt[threadIdx.x]);
     if (threadIdx.x == 0) {
         tile_sums[blockIdx.x] = s_sum;
-        atomicMax(&meta->max_nnz, s_max);
-        if (s_pool) atomicAdd(&meta->gnum_pool, s_pool);
+        atomicMax(&meta->max_nnz.v, s_max);
+        if (s_pool) atomicAdd(&meta->gnum_pool.v, s_pool);
     }
 }
 
@@ -228,7 +232,7 @@ __global__ __launch_bounds__(kRowThreads) void k_scan_apply(ProbSet<V> ps, BinTa
                                                             int2* __restrict__ items,
                                                             long long* __restrict__ gpool_off,
                                                             long long* __restrict__ inst_nnz, int after_fail) {
-    if (!after_fail && meta->fail_count > 0) return;
+    if (!after_fail && meta->fail_count.v > 0) return;
     constexpr int kPer = kTileRows / kRowThreads;  // consecutive rows per thread
     constexpr int kWaves = kRowThreads / kWave;
     __shared__ long long s_wave[kWaves];
@@ -300,17 +304,17 @@ __global__ __launch_bounds__(kRowThreads) void k_scan_apply(ProbSet<V> ps, BinTa
         if (run > INT_MAX) flags |= kFlagNnzOverflow;
         P.c_rpt[P.m] = static_cast<int>(run > INT_MAX ? INT_MAX : run);
         inst_nnz[t.inst] = run;
-        if (ps.n_inst == 1) meta->nnz_total = run;
+        if (ps.n_inst == 1) meta->nnz_total.v = run;
     }
     if (flags) atomicOr(&s_flags, flags);
     __syncthreads();
 
     if (threadIdx.x < num.n && s_cnt[threadIdx.x] > 0) {
         int off = 0;
-        for (int b = 0; b < static_cast<int>(threadIdx.x); ++b) off += meta->num_count[b];
-        s_base[threadIdx.x] = off + atomicAdd(&meta->num_cursor[threadIdx.x], s_cnt[threadIdx.x]);
+        for (int b = 0; b < static_cast<int>(threadIdx.x); ++b) off += meta->num_count[b].v;
+        s_base[threadIdx.x] = off + atomicAdd(&meta->num_cursor[threadIdx.x].v, s_cnt[threadIdx.x]);
     }
-    if (threadIdx.x == 0 && s_flags) atomicOr(&meta->flags, s_flags);
+    if (threadIdx.x == 0 && s_flags) atomicOr(&meta->flags.v, s_flags);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
@@ -319,21 +323,29 @@ __global__ __launch_bounds__(kRowThreads) void k_scan_apply(ProbSet<V> ps, BinTa
         items[pos] = make_int2(t.inst, t.row0 + r0 + j);
         if (num.variant[bin[j]] == kNumGlobalVariant)
             gpool_off[pos] = static_cast<long long>(
-                atomicAdd(&meta->gnum_cursor, static_cast<unsigned long long>(global_table_slots(v[j]))));
+                atomicAdd(&meta->gnum_cursor.v, static_cast<unsigned long long>(global_table_slots(v[j]))));
     }
 }
 
-// Total NIP of A . B for the FLOP metric (nip.cuh:19-49), one thread per row.
+// Total NIP of A . B for the FLOP metric (nip.cuh:19-49): a tile of rows
+// per workgroup, a thread per row, one atomic per workgroup.
 __global__ __launch_bounds__(kRowThreads) void k_total_nip(const int32_t* __restrict__ a_rpt,
                                                            const int32_t* __restrict__ a_col,
                                                            const int32_t* __restrict__ b_rpt, int m,
                                                            unsigned long long* out) {
-    const int row = blockIdx.x * kRowThreads + threadIdx.x;
+    __shared__ unsigned long long s_total;
+    if (threadIdx.x == 0) s_total = 0;
+    __syncthreads();
     long long s = 0;
-    if (row < m)
-        for (int p = a_rpt[row]; p < a_rpt[row + 1]; ++p) s += b_rpt[a_col[p] + 1] - b_rpt[a_col[p]];
+    for (int j = 0; j < kTileRows / kRowThreads; ++j) {
+        const int row = blockIdx.x * kTileRows + j * kRowThreads + threadIdx.x;
+        if (row < m)
+            for (int p = a_rpt[row]; p < a_rpt[row + 1]; ++p) s += b_rpt[a_col[p] + 1] - b_rpt[a_col[p]];
+    }
     s = wave_sum64(s);
-    if (lane_id() == 0 && s) atomicAdd(out, static_cast<unsigned long long>(s));
+    if (lane_id() == 0 && s) atomicAdd(&s_total, static_cast<unsigned long long>(s));
+    __syncthreads();
+    if (threadIdx.x == 0 && s_total) atomicAdd(out, s_total);
 }
 
 // ----------------------------------------------------------------- launchers
@@ -341,12 +353,12 @@ __global__ __launch_bounds__(kRowThreads) void k_total_nip(const int32_t* __rest
 void launch_total_nip(const int32_t* a_rpt, const int32_t* a_col, const int32_t* b_rpt, int m,
                       unsigned long long* out, hipStream_t s) {
     if (m <= 0) return;
-    hipLaunchKernelGGL(k_total_nip, dim3((m + kRowThreads - 1) / kRowThreads), dim3(kRowThreads), 0, s, a_rpt,
+    hipLaunchKernelGGL(k_total_nip, dim3((m + kTileRows - 1) / kTileRows), dim3(kRowThreads), 0, s, a_rpt,
                        a_col, b_rpt, m, out);
 }
 
 void launch_clear(DevMeta* meta, int from_word, hipStream_t s) {
-    hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, s, meta, from_word);
+    hipLaunchKernelGGL(k_clear, dim3(1), dim3(256), 0, s, meta, from_word);
 }
 
 template <typename V>

@@ -71,7 +71,7 @@ __global__ __launch_bounds__(kMultiThreads) void k_sym_multi(ProbSet<V> ps, cons
     if (threadIdx.x == 0) {
         int off = 0, units = 0;
         for (int b = 0; b < n_multi; ++b) {
-            const int c = meta->sym_count[b];
+            const int c = meta->sym_count[b].v;
             s_off[b] = off;
             s_cnt[b] = c;
             s_unit[b] = units;
@@ -127,8 +127,8 @@ __global__ __launch_bounds__(1024) void k_sym_max(ProbSet<V> ps, const int2* __r
     __shared__ int s_tmp[WG / kWave];
     __shared__ int s_n;
     int off = 0;
-    for (int b = 0; b < bin; ++b) off += meta->sym_count[b];
-    const int count = meta->sym_count[bin];
+    for (int b = 0; b < bin; ++b) off += meta->sym_count[b].v;
+    const int count = meta->sym_count[bin].v;
     for (int u = blockIdx.x; u < count; u += gridDim.x) {
         lds_fill<T, WG>(tab, kEmpty, threadIdx.x);
         if (threadIdx.x == 0) s_n = 0;
@@ -157,10 +157,10 @@ __global__ __launch_bounds__(1024) void k_sym_max(ProbSet<V> ps, const int2* __r
             if (n <= kSymMaxThreshold) {
                 P.c_rpt[it.y] = n;
             } else {
-                const int f = atomicAdd(&meta->fail_count, 1);
+                const int f = atomicAdd(&meta->fail_count.v, 1);
                 fail_items[f] = it;
                 fail_off[f] = static_cast<long long>(atomicAdd(
-                    &meta->fail_pool, static_cast<unsigned long long>(global_table_slots(P.nip[it.y]))));
+                    &meta->fail_pool.v, static_cast<unsigned long long>(global_table_slots(P.nip[it.y]))));
             }
         }
         __syncthreads();

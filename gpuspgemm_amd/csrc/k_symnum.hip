@@ -138,7 +138,7 @@ __device__ __forceinline__ void symnum_rows(const ProbSet<V>& ps, const int2* __
 
     int w = 0;
     if (active) {
-        if (lr == 0 && cn != found) atomicOr(&meta->flags, kFlagMismatch);
+        if (lr == 0 && cn != found) atomicOr(&meta->flags.v, kFlagMismatch);
         w = min(n, cn);
     }
     if constexpr (L <= kWave) {
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(1024) void k_symnum_global(ProbSet<V> ps, const int
     }
     __threadfence();
     __syncthreads();
-    if (threadIdx.x == 0 && s_n != nnz) atomicOr(&meta->flags, kFlagMismatch);
+    if (threadIdx.x == 0 && s_n != nnz) atomicOr(&meta->flags.v, kFlagMismatch);
     const int n = min(s_n, nnz);
 
     // sort chunks in LDS, staging them in the (now free) table
@@ -322,7 +322,8 @@ static int occupancy_variant() {
     return n > 0 ? n : 1;
 }
 
-#define SPG_NUM_VARIANTS(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13)
+#define SPG_NUM_VARIANTS(X) \
+    X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18)
 
 template <typename V>
 hipError_t launch_symnum_bin(int variant, const ProbSet<V>& ps, const int2* items, int item_off, int count,

@@ -28,8 +28,8 @@ namespace spg {
 constexpr int kWave = 64;          // CDNA wavefront width
 constexpr int kEmpty = -1;         // empty hash slot (columns are >= 0)
 constexpr int kMaxBins = 16;
-constexpr int kTileRows = 1024;    // rows per tile of the row-parallel passes
-constexpr int kRowThreads = 256;   // threads of the row-parallel passes
+constexpr int kTileRows = 4096;    // rows per tile of the row-parallel passes
+constexpr int kRowThreads = 1024;  // threads of the row-parallel passes (few workgroups: few atomics per counter)
 constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups dealt round-robin
 constexpr int kMultiThreads = 256; // workgroup size of the multi-bin kernels
 
@@ -99,26 +99,32 @@ enum MetaFlags : int32_t {
     kFlagMismatch = 4,      // fused pass found a different nnz than the count pass
 };
 
+// A device counter on a 128-byte line of its own.  Counters are updated
+// atomically by many workgroups; atomics from all XCDs to one line are
+// serialised (about 10 ns each), so no two counters share a line.
+template <typename T>
+struct alignas(128) Counter {
+    T v;
+};
+
 // Device counters of one multiply (zeroed by k_clear).  The second group
 // belongs to the scan / fused-bin stage and is cleared again on its own
 // when overflowed rows force that stage to run twice.
 struct DevMeta {
-    int32_t sym_count[kMaxBins];
-    int32_t sym_cursor[kMaxBins];
-    int32_t fail_count;
-    int32_t flags;
-    int32_t max_nip;
-    int32_t pad0;
-    unsigned long long total_nip;
-    unsigned long long fail_pool;    // int32 slots of the global tables of overflowed rows
+    Counter<int32_t> sym_count[kMaxBins];
+    Counter<int32_t> sym_cursor[kMaxBins];
+    Counter<int32_t> fail_count;
+    Counter<int32_t> flags;
+    Counter<int32_t> max_nip;
+    Counter<unsigned long long> total_nip;
+    Counter<unsigned long long> fail_pool;  // int32 slots of the global tables of overflowed rows
     // ---- scan stage
-    int32_t num_count[kMaxBins];
-    int32_t num_cursor[kMaxBins];
-    int32_t max_nnz;
-    int32_t pad1;
-    unsigned long long gnum_pool;    // int32 slots needed by the global fused bin
-    unsigned long long gnum_cursor;  // slots handed out so far
-    long long nnz_total;             // nnz(C) of a single product
+    Counter<int32_t> num_count[kMaxBins];
+    Counter<int32_t> num_cursor[kMaxBins];
+    Counter<int32_t> max_nnz;
+    Counter<unsigned long long> gnum_pool;    // int32 slots needed by the global fused bin
+    Counter<unsigned long long> gnum_cursor;  // slots handed out so far
+    Counter<long long> nnz_total;             // nnz(C) of a single product
 };
 constexpr int kMetaScanWord = static_cast<int>(offsetof(DevMeta, num_count) / sizeof(int32_t));
 

@@ -34,17 +34,19 @@ constexpr int kSymMultiLds = 36 * 1024;  // bytes: max over variants 0..8 of r *
 
 // Fused sorted-columns + numeric variants, one kernel each: all use 256
 // threads except 9 (a 1024-thread workgroup per row); rows above every
-// capacity use global-memory tables (variant 14).  A row whose products fit
+// capacity use global-memory tables (variant 19).  A row whose products fit
 // in one chunk and in lanes * pmax keeps them in registers between the two
-// halves.  Variants 10..13 are the small capacities with more lanes per row.
+// halves.  Variants 10..18 cover the small capacities with 2 to 32 lanes
+// per row; the tuner picks among them per bin.
 constexpr NumVariant kNumVariants[] = {
     {256, 256, 8, 4, 16},   {256, 128, 16, 4, 16}, {256, 64, 32, 8, 16},  {256, 32, 64, 16, 16},
     {256, 16, 128, 32, 16}, {256, 4, 128, 64, 12}, {256, 4, 512, 64, 16}, {256, 1, 1024, 256, 8},
     {256, 1, 2048, 256, 8}, {1024, 1, 8192, 1024, 4}, {256, 32, 8, 8, 2}, {256, 16, 16, 16, 2},
-    {256, 16, 32, 16, 4},   {256, 8, 64, 32, 4},
+    {256, 16, 32, 16, 4},   {256, 8, 64, 32, 4},   {256, 64, 8, 8, 4},    {256, 128, 8, 4, 8},
+    {256, 32, 16, 8, 4},    {256, 64, 16, 8, 8},   {256, 32, 32, 16, 8},
 };
-constexpr int kNumLdsVariants = 14;  // variants with an LDS kernel (0..13)
-constexpr int kNumGlobalVariant = 14;
+constexpr int kNumLdsVariants = 19;  // variants with an LDS kernel (0..18)
+constexpr int kNumGlobalVariant = 19;
 
 constexpr int kGlobalSortChunk = 4096;  // LDS chunk of the global-bin column sort
 
