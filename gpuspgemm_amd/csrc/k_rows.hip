@@ -33,7 +33,7 @@ __global__ __launch_bounds__(kRowThreads) void k_nip(ProbSet<V> ps, BinTable sym
     __syncthreads();
 
     const Tile t = ps.tile(blockIdx.x);
-    const Prob<V> P = ps.get(t.inst);
+    const GProb<V> P = ps.get(t.inst);
     constexpr int kPer = kTileRows / kRowThreads;
     constexpr int kShort = 4;  // entries of a row read without a loop
     // all loads of a thread's rows are issued together: row pointers, then
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kRowThreads) void k_scatter(ProbSet<V> ps, BinTable
     if (threadIdx.x < kMaxBins) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     const Tile t = ps.tile(blockIdx.x);
-    const Prob<V> P = ps.get(t.inst);
+    const GProb<V> P = ps.get(t.inst);
     int bin[kTileRows / kRowThreads];
     int rank[kTileRows / kRowThreads];
 #pragma unroll
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(kRowThreads) void k_scan_count(ProbSet<V> ps, BinTa
     }
     __syncthreads();
     const Tile t = ps.tile(blockIdx.x);
-    const Prob<V> P = ps.get(t.inst);
+    const GProb<V> P = ps.get(t.inst);
     long long sum = 0;
     unsigned long long pool = 0;
     int mx = 0;
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(kRowThreads) void k_scan_apply(ProbSet<V> ps, BinTa
     __shared__ int s_flags;
 
     const Tile t = ps.tile(blockIdx.x);
-    const Prob<V> P = ps.get(t.inst);
+    const GProb<V> P = ps.get(t.inst);
     if (threadIdx.x < kMaxBins) s_cnt[threadIdx.x] = 0;
     if (threadIdx.x == 0) s_flags = 0;
 

@@ -28,7 +28,7 @@ __device__ __forceinline__ void sym_rows(const ProbSet<V>& ps, const int2* __res
     const int idx = first + slot;
     const bool active = idx < end;
     const int2 it = active ? items[idx] : make_int2(0, 0);
-    const Prob<V> P = ps.get(it.x);
+    const GProb<V> P = ps.get(it.x);
     int a0 = 0, a1 = 0;
     if (active) {
         a0 = P.a_rpt[it.y];
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(1024) void k_sym_max(ProbSet<V> ps, const int2* __r
         if (threadIdx.x == 0) s_n = 0;
         __syncthreads();
         const int2 it = items[off + u];
-        const Prob<V> P = ps.get(it.x);
+        const GProb<V> P = ps.get(it.x);
         const int a0 = P.a_rpt[it.y];
         const int a1 = P.a_rpt[it.y + 1];
         bool over = false;
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(1024) void k_sym_global(ProbSet<V> ps, const int2* 
     __shared__ Meta cm;
     __shared__ int s_tmp[WG / kWave];
     const int2 it = fail_items[blockIdx.x];
-    const Prob<V> P = ps.get(it.x);
+    const GProb<V> P = ps.get(it.x);
     const long long slots = global_table_slots(P.nip[it.y]);
     const int bits = __builtin_ctzll(static_cast<unsigned long long>(slots));
     int* tab = pool + fail_off[blockIdx.x];

@@ -99,7 +99,7 @@ __device__ __forceinline__ void symnum_rows(const ProbSet<V>& ps, const int2* __
     const int idx = first + slot;
     const bool active = idx < end;
     const int2 it = active ? items[idx] : make_int2(0, 0);
-    const Prob<V> Pr = ps.get(it.x);
+    const GProb<V> Pr = ps.get(it.x);
     int a0 = 0, a1 = 0, nip = 0, c0 = 0, cn = 0;
     if (active) {
         a0 = Pr.a_rpt[it.y];
@@ -191,7 +191,8 @@ __global__ __launch_bounds__(kNumVariants[VV].wg) void k_symnum_bin(ProbSet<V> p
 }
 
 // Lower bound in a sorted global list (values read coherently).
-__device__ __forceinline__ int lower_bound_global(const int* cols, int n, int key) {
+template <typename Ptr>
+__device__ __forceinline__ int lower_bound_global(Ptr cols, int n, int key) {
     int lo = 0, hi = n;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
@@ -223,15 +224,15 @@ __global__ __launch_bounds__(1024) void k_symnum_global(ProbSet<V> ps, const int
     __shared__ int s_tmp[WG / kWave];
     __shared__ int s_n;
     const int2 it = items[item_off + blockIdx.x];
-    const Prob<V> P = ps.get(it.x);
+    const GProb<V> P = ps.get(it.x);
     const int c0 = P.c_rpt[it.y];
     const int nnz = P.c_rpt[it.y + 1] - c0;
     const long long slots = global_table_slots(nnz);
     const int bits = __builtin_ctzll(static_cast<unsigned long long>(slots));
     const uint32_t mask = static_cast<uint32_t>(slots - 1);
     int* tab = pool + gpool_off[item_off + blockIdx.x];
-    int* ccol = P.c_col + c0;
-    V* cval = P.c_val + c0;
+    auto ccol = P.c_col + c0;
+    auto cval = P.c_val + c0;
     for (long long j = threadIdx.x; j < slots; j += WG) tab[j] = kEmpty;
     for (int j = threadIdx.x; j < nnz; j += WG) cval[j] = V(0);
     if (threadIdx.x == 0) s_n = 0;

@@ -52,9 +52,48 @@ struct Prob {
     int32_t n;
 };
 
+// Global address space.  Data pointers reach the kernels inside structs, so
+// the compiler sees them as generic and would emit flat instructions, which
+// also count against the LDS wait counter: every wait for an LDS result
+// would then wait for all outstanding memory loads too.  Device code reads
+// and writes through pointers cast to this address space instead.
+#ifndef SPG_GLOBAL
+#define SPG_GLOBAL __attribute__((address_space(1)))
+#endif
+
+template <typename T>
+__device__ __forceinline__ T SPG_GLOBAL* to_global(T* p) {
+    return (T SPG_GLOBAL*)p;
+}
+
+// Device view of one product: the pointers of Prob in the global address space.
+template <typename V>
+struct GProb {
+    const int32_t SPG_GLOBAL* a_rpt;
+    const int32_t SPG_GLOBAL* a_col;
+    const V SPG_GLOBAL* a_val;
+    const int32_t SPG_GLOBAL* b_rpt;
+    const int32_t SPG_GLOBAL* b_col;
+    const V SPG_GLOBAL* b_val;
+    int32_t SPG_GLOBAL* c_rpt;
+    int32_t SPG_GLOBAL* c_col;
+    V SPG_GLOBAL* c_val;
+    int32_t SPG_GLOBAL* nip;
+    int32_t m;
+    int32_t n;
+};
+
+// (R: a Prob lvalue in any address space)
+template <typename V, typename R>
+__device__ __forceinline__ GProb<V> global_view(R&& p) {
+    return GProb<V>{to_global(p.a_rpt), to_global(p.a_col), to_global(p.a_val), to_global(p.b_rpt),
+                    to_global(p.b_col), to_global(p.b_val), to_global(p.c_rpt), to_global(p.c_col),
+                    to_global(p.c_val), to_global(p.nip),   p.m,                p.n};
+}
+
 // Tile of a row-parallel pass: instance, first row, row count, index of the
-// instance's first tile.
-struct Tile {
+// instance's first tile (read on the device as one int4).
+struct alignas(16) Tile {
     int32_t inst, row0, rows, first_tile;
 };
 
@@ -68,13 +107,18 @@ struct ProbSet {
     int32_t n_inst;
     int32_t n_tiles;
 
-    __device__ __forceinline__ Prob<V> get(int i) const { return n_inst == 1 ? p0 : many[i]; }
+    __device__ __forceinline__ GProb<V> get(int i) const {
+        if (n_inst == 1) return global_view<V>(p0);
+        return global_view<V>(to_global(many)[i]);
+    }
     __device__ __forceinline__ Tile tile(int t) const {
         if (n_inst == 1) {
             const int r0 = t * kTileRows;
             return Tile{0, r0, min(kTileRows, p0.m - r0), 0};
         }
-        return tiles[t];
+        typedef int32_t Words __attribute__((vector_size(16)));
+        const Words v = to_global(reinterpret_cast<const Words*>(tiles))[t];
+        return Tile{v[0], v[1], v[2], v[3]};
     }
 };
 

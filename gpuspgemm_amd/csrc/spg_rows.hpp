@@ -86,7 +86,7 @@ struct ChunkMeta {
 // Load entries [ebase, min(ebase + EC, a1)) of A's row into `cm`; returns
 // the chunk's product count.  Each lane loads EC/L consecutive entries.
 template <typename V, int L, int EC, bool VALS>
-__device__ __forceinline__ int load_chunk(const Prob<V>& P, int ebase, int a1, int lr,
+__device__ __forceinline__ int load_chunk(const GProb<V>& P, int ebase, int a1, int lr,
                                           ChunkMeta<V, EC, VALS>* cm, int* s_tmp) {
     static_assert(EC % L == 0, "chunk must be a multiple of the group");
     constexpr int PER = EC / L;
@@ -134,7 +134,7 @@ __device__ __forceinline__ int chunk_find(const int* off, int p) {
 // Fetch up to PMAX products of a chunk: slot j holds product
 // base + j*L + lr.  ok[j] tells which slots are real.
 template <typename V, int L, int EC, int PMAX, bool VALS>
-__device__ __forceinline__ void fetch_products(const Prob<V>& P, const ChunkMeta<V, EC, VALS>* cm, int total,
+__device__ __forceinline__ void fetch_products(const GProb<V>& P, const ChunkMeta<V, EC, VALS>* cm, int total,
                                                int base, int lr, int (&key)[PMAX], V (&prod)[PMAX],
                                                bool (&ok)[PMAX]) {
 #pragma unroll
@@ -154,7 +154,7 @@ __device__ __forceinline__ void fetch_products(const Prob<V>& P, const ChunkMeta
 
 // A lane that owns its row keeps the chunk metadata in registers.
 template <typename V, int EC, bool VALS>
-__device__ __forceinline__ int load_chunk_lane(const Prob<V>& P, int ebase, int a1, ChunkMeta<V, EC, VALS>& cm) {
+__device__ __forceinline__ int load_chunk_lane(const GProb<V>& P, int ebase, int a1, ChunkMeta<V, EC, VALS>& cm) {
     int run = 0;
     int bs[EC], len[EC];
 #pragma unroll
@@ -180,7 +180,7 @@ __device__ __forceinline__ int load_chunk_lane(const Prob<V>& P, int ebase, int 
 }
 
 template <typename V, int EC, int PMAX, bool VALS>
-__device__ __forceinline__ void fetch_products_lane(const Prob<V>& P, const ChunkMeta<V, EC, VALS>& cm, int total,
+__device__ __forceinline__ void fetch_products_lane(const GProb<V>& P, const ChunkMeta<V, EC, VALS>& cm, int total,
                                                     int base, int (&key)[PMAX], V (&prod)[PMAX], bool (&ok)[PMAX]) {
 #pragma unroll
     for (int j = 0; j < PMAX; ++j) {
@@ -209,7 +209,7 @@ __device__ __forceinline__ void fetch_products_lane(const Prob<V>& P, const Chun
 // Call g(key[], prod[], ok[]) for every round of up to PMAX products per
 // lane of A's entries [a0, a1).
 template <typename V, int L, int EC, int PMAX, bool VALS, typename G>
-__device__ __forceinline__ void for_each_round(const Prob<V>& P, int a0, int a1, int lr,
+__device__ __forceinline__ void for_each_round(const GProb<V>& P, int a0, int a1, int lr,
                                                ChunkMeta<V, EC, VALS>* cm, int* s_tmp, G&& g) {
     for (int ebase = a0; ebase < a1; ebase += EC) {
         int key[PMAX];
@@ -235,7 +235,7 @@ __device__ __forceinline__ void for_each_round(const Prob<V>& P, int a0, int a1,
 
 // Call f(key, prod) for every product of A's entries [a0, a1).
 template <typename V, int L, int EC, int PMAX, bool VALS, typename F>
-__device__ __forceinline__ void for_each_product(const Prob<V>& P, int a0, int a1, int lr,
+__device__ __forceinline__ void for_each_product(const GProb<V>& P, int a0, int a1, int lr,
                                                  ChunkMeta<V, EC, VALS>* cm, int* s_tmp, F&& f) {
     for_each_round<V, L, EC, PMAX, VALS>(P, a0, a1, lr, cm, s_tmp,
                                          [&](const int(&key)[PMAX], const V(&prod)[PMAX], const bool(&ok)[PMAX]) {

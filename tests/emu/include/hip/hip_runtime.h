@@ -27,6 +27,7 @@
 #define __host__
 #define __shared__ static
 #define __forceinline__ inline
+#define SPG_GLOBAL  // host threads have one address space
 #define __noinline__
 #define __restrict__
 #define __align__(n) alignas(n)
