@@ -189,6 +189,152 @@ def cpu_baseline(A, nip: int, budget_s: float) -> dict:
                       f"({threads} threads), {t_total:.1f} s"}
 
 
+def _batched_pair(i: int):
+    from gpuspgemm_amd import generators as gen
+    return gen.batched_instance(i, seed=3)
+
+
+def _pool_map(fn, items):
+    import multiprocessing as mp
+    workers = max(1, min(16, os.cpu_count() or 1))
+    with mp.get_context("fork").Pool(workers) as pool:
+        return pool.map(fn, items, chunksize=4)
+
+
+def run_batched(n_inst: int, reps: int) -> dict:
+    """Batched form (512 independent products in one call of
+    spg_multiply_batched) against the plain per-call loop over the same
+    products: intermediate products per second of each."""
+    import gpuspgemm_amd as g
+    g.load_library()
+    pairs = _pool_map(_batched_pair, range(n_inst))
+    dev = [(a.to_device(), b.to_device()) for a, b in pairs]
+    nip = sum(g.get_nip(da, db) for da, db in dev)
+    As, Bs = [a for a, _ in dev], [b for _, b in dev]
+
+    def batched():
+        t0 = time.perf_counter()
+        cs = g.proposal_batched(As, Bs)
+        t = time.perf_counter() - t0
+        for c in cs:
+            c.free()
+        return t
+
+    def loop():
+        t0 = time.perf_counter()
+        cs = [g.proposal(a, b) for a, b in dev]
+        t = time.perf_counter() - t0
+        for c in cs:
+            c.free()
+        return t
+
+    batched(), loop()  # warm the pools
+    tb = min(batched() for _ in range(reps))
+    tl = min(loop() for _ in range(reps))
+    for a, b in dev:
+        a.free()
+        b.free()
+    rows = int(sum(p[0].m for p in pairs))
+    return {"config": {"workload": "batched", "instances": n_inst, "rows": rows, "row_length": "Geometric(1/6)",
+                       "rmat_rows": "1/3", "seed": 3}, "dtype": "f64", "nip": int(nip),
+            "batched_ms": round(tb * 1e3, 3), "loop_ms": round(tl * 1e3, 3),
+            "batched_products_per_s": nip / tb, "loop_products_per_s": nip / tl, "speedup": tl / tb,
+            "timing": f"best of {reps}, wall clock around the whole call (outputs allocated inside)"}
+
+
+def _served_request(k: int) -> dict:
+    from gpuspgemm_amd import generators as gen
+    r = gen.rand_unit(4, 1, np.arange(4 * k, 4 * k + 4))
+    dtype = np.float64 if r[0] < 0.7 else np.float32
+    kind = ["uniform8", "powerlaw", "stencil", "batched"][min(3, int(r[1] * 4))]
+    m = int(round(2.0 ** (10 + 10 * r[2])))
+    if kind == "uniform8":
+        A = gen.uniform8(m=m, seed=100 + k)
+        B = A
+    elif kind == "powerlaw":
+        A = gen.powerlaw(m=m, seed=100 + k)
+        B = A
+    elif kind == "stencil":
+        A = gen.stencil27(g=max(2, int(round(m ** (1 / 3)))), seed=100 + k)
+        B = A
+    else:
+        A, B = gen.batched_instance(k, seed=4, m=m)
+    return {"A": A.astype(dtype), "B": B.astype(dtype), "kind": kind, "m": A.m, "dtype": np.dtype(dtype).name}
+
+
+def run_served(n_req: int, load: float, budget_gb: float, max_batch: int) -> dict:
+    """Served form: Poisson arrivals of mixed requests (70 % FP64, 30 % FP32;
+    generator drawn from uniform8 / power-law / stencil / batched instance;
+    rows log-uniform in [2^10, 2^20]) at an offered load of `load` times the
+    rate the serial plain path sustains; operands are device-resident.
+    Latency is from submit to completion as the server records it."""
+    import threading
+    import gpuspgemm_amd as g
+    g.load_library()
+    reqs = _pool_map(_served_request, range(n_req))
+    dev = [(q["A"].to_device(), q["B"].to_device()) for q in reqs]
+    nips = [g.get_nip(a, b) for a, b in dev]
+    # isolated (serial plain path) latency of every request
+    for a, b in dev[: min(8, n_req)]:
+        g.proposal(a, b).free()
+    iso = []
+    t0 = time.perf_counter()
+    for a, b in dev:
+        t1 = time.perf_counter()
+        g.proposal(a, b).free()
+        iso.append(time.perf_counter() - t1)
+    serial_s = time.perf_counter() - t0
+    rate = load * n_req / serial_s  # requests per second offered
+    gaps = -np.log(1.0 - gen_unit(5, n_req)) / rate
+    arrivals = np.cumsum(gaps)
+    lat = [0.0] * n_req
+    with g.Server(budget_bytes=int(budget_gb * (1 << 30)), max_batch=max_batch) as srv:
+        tickets = [None] * n_req
+        done = threading.Event()
+
+        def waiter():
+            for i in range(n_req):
+                while tickets[i] is None:
+                    time.sleep(0.0001)
+                c, ms = srv.wait(tickets[i])
+                lat[i] = ms * 1e-3
+                c.free()
+            done.set()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        start = time.perf_counter()
+        for i in range(n_req):
+            now = time.perf_counter() - start
+            if arrivals[i] > now:
+                time.sleep(arrivals[i] - now)
+            tickets[i] = srv.submit(*dev[i])
+        th.join()
+        makespan = time.perf_counter() - start
+        st = srv.stats()
+    for a, b in dev:
+        a.free()
+        b.free()
+    lat = np.array(lat)
+    iso = np.array(iso)
+    total = float(sum(nips))
+    return {"config": {"workload": "served", "requests": n_req, "rows": "log-uniform [2^10, 2^20]",
+                       "mix": "70% f64 / 30% f32; uniform8, power-law, stencil, batched instance", "seed": 4,
+                       "offered_load_req_per_s": round(rate, 2), "load_vs_serial": load,
+                       "budget_gb": budget_gb, "max_batch": max_batch},
+            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "isolated_p50_ms": float(np.percentile(iso, 50) * 1e3),
+            "isolated_p99_ms": float(np.percentile(iso, 99) * 1e3),
+            "p99_vs_isolated_p99": float(np.percentile(lat, 99) / np.percentile(iso, 99)),
+            "products_per_s": total / makespan, "serial_products_per_s": total / serial_s,
+            "vs_serial": (total / makespan) / (total / serial_s), "server": st}
+
+
+def gen_unit(stream: int, n: int) -> np.ndarray:
+    from gpuspgemm_amd import generators as gen
+    return gen.rand_unit(4, stream, np.arange(n))
+
+
 def copy_calibration(mbytes: int, reps: int = 5) -> None:
     """Device-to-device copies of a known size, for scaling rocprofv3's
     FETCH_SIZE / WRITE_SIZE counters to bytes (run under --pmc)."""
@@ -221,6 +367,15 @@ def main(argv=None) -> int:
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run device copies of MB MiB (counter calibration)")
+    ap.add_argument("--form", default="all", choices=["all", "plain", "batched", "served"],
+                    help="all: the plain line with the batched and served forms attached")
+    ap.add_argument("--instances", type=int, default=512, help="batched form: products per batch")
+    ap.add_argument("--requests", type=int, default=160, help="served form: requests")
+    ap.add_argument("--load", type=float, default=1.0, help="served form: offered load / serial rate")
+    ap.add_argument("--sat-load", type=float, default=4.0,
+                    help="served form, throughput run: offered load / serial rate")
+    ap.add_argument("--budget-gb", type=float, default=16.0, help="served form: output memory budget")
+    ap.add_argument("--max-batch", type=int, default=32, help="served form: largest batch")
     args = ap.parse_args(argv)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -230,6 +385,12 @@ def main(argv=None) -> int:
               "see DESIGN.md for how it would shard)", file=sys.stderr)
         return 2
     dtype = np.float64 if args.dtype == "f64" else np.float32
+    if args.form == "batched":
+        print(json.dumps(run_batched(args.instances, 3)), flush=True)
+        return 0
+    if args.form == "served":
+        print(json.dumps(run_served(args.requests, args.load, args.budget_gb, args.max_batch)), flush=True)
+        return 0
 
     A, cfg = make_workload(args.workload)
     res = run_gpu(A, args.steps, args.warmup, args.pause, dtype)
@@ -256,6 +417,10 @@ def main(argv=None) -> int:
             "roofline": sout["roofline"], "nip": sout["nip"], "nnz_c": sout["nnz_c"],
         }
         del S
+    if args.form == "all":
+        line["batched"] = run_batched(args.instances, 3)
+        line["served"] = run_served(args.requests, args.load, args.budget_gb, args.max_batch)
+        line["served_saturated"] = run_served(args.requests, args.sat_load, args.budget_gb, args.max_batch)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(A.astype(dtype), out["nip"], args.cpu_budget)
     else:

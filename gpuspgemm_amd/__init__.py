@@ -27,6 +27,7 @@ from .csr import DeviceCSR, HostCSR, _dtype_code
 __all__ = [
     "HostCSR", "DeviceCSR", "SpgError", "proposal", "proposal_batched", "get_nip",
     "device_info", "set_tuning", "get_tuning", "get_variants", "variant_list", "profile", "load_library",
+    "Server",
 ]
 
 
@@ -142,3 +143,53 @@ class profile:
     def __exit__(self, *exc):
         self.collect()
         _lib.check(self.lib.spg_set_profiling(0))
+
+
+class Server:
+    """Served form: a request loop in the library (spg_server_*) with
+    continuous batching and admission by exact output memory.
+
+        with Server(budget_bytes=8 << 30, max_batch=64) as srv:
+            t = srv.submit(dA, dB)          # device-resident operands
+            C, latency_ms = srv.wait(t)     # C: DeviceCSR owned by the caller
+
+    Operands are kept referenced until their request is waited for."""
+
+    def __init__(self, budget_bytes: int, max_batch: int = 64):
+        self.lib = _lib.load()
+        self.handle = ctypes.c_void_p()
+        _lib.check(self.lib.spg_server_create(int(budget_bytes), int(max_batch), ctypes.byref(self.handle)))
+        self._keep: dict[int, tuple] = {}
+
+    def submit(self, A: DeviceCSR, B: DeviceCSR) -> int:
+        if A.dtype != B.dtype:
+            raise TypeError("A and B must have the same value type")
+        t = ctypes.c_int64()
+        _lib.check(self.lib.spg_server_submit(self.handle, ctypes.byref(A.raw), ctypes.byref(B.raw),
+                                              _dtype_code(A.dtype), ctypes.byref(t)))
+        self._keep[t.value] = (A, B)
+        return t.value
+
+    def wait(self, ticket: int) -> tuple[DeviceCSR, float]:
+        out = _lib.SpgCsr()
+        lat = ctypes.c_double()
+        A, _ = self._keep.pop(ticket)
+        _lib.check(self.lib.spg_server_wait(self.handle, ticket, ctypes.byref(out), ctypes.byref(lat)))
+        return DeviceCSR(out, A.dtype, library_owned=True), lat.value
+
+    def stats(self) -> dict:
+        v = [ctypes.c_int64() for _ in range(5)]
+        _lib.check(self.lib.spg_server_stats(self.handle, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("batches", "served", "max_batch", "deferred", "peak_bytes"), (x.value for x in v)))
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.check(self.lib.spg_server_destroy(self.handle))
+            self.handle = ctypes.c_void_p()
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -58,6 +58,12 @@ _SIG = {
     "spg_last_error": (ctypes.c_char_p, []),
     "spg_device_info_get": (_I32, [ctypes.POINTER(SpgDeviceInfo)]),
     "spg_set_tuning": (_I32, [_I32, ctypes.c_char_p]),
+    "spg_server_create": (_I32, [ctypes.c_size_t, _I32, ctypes.POINTER(ctypes.c_void_p)]),
+    "spg_server_submit": (_I32, [_P, ctypes.POINTER(SpgCsr), ctypes.POINTER(SpgCsr), _I32,
+                                 ctypes.POINTER(ctypes.c_int64)]),
+    "spg_server_wait": (_I32, [_P, ctypes.c_int64, ctypes.POINTER(SpgCsr), ctypes.POINTER(ctypes.c_double)]),
+    "spg_server_stats": (_I32, [_P] + [ctypes.POINTER(ctypes.c_int64)] * 5),
+    "spg_server_destroy": (_I32, [_P]),
     "spg_get_tuning": (_I32, [_I32, ctypes.c_char_p, ctypes.c_size_t]),
     "spg_get_variants": (_I32, [_I32, ctypes.c_char_p, ctypes.c_size_t]),
     "spg_set_profiling": (_I32, [_I32]),

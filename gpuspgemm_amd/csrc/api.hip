@@ -18,6 +18,12 @@
 #include "../../include/gpuspgemm.h"
 
 #include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <thread>
+#include <unordered_map>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -65,10 +71,29 @@ struct ProfRec {
     double ms;
 };
 
+// The device resources of one multiply in flight: a main stream plus aux
+// streams that the fused-pass launches are spread over, events, a scratch
+// arena that only grows, and pinned host memory for the read-backs.  The
+// context owns one lane for the plain calls; a server owns lanes of its own
+// so that the symbolic phase of one batch overlaps the fused pass of the
+// batch before it.
+struct Lane {
+    static constexpr int kStreams = 4;
+    hipStream_t s = nullptr;
+    hipStream_t aux[kStreams - 1] = {};
+    hipEvent_t ev_ready = nullptr;
+    hipEvent_t ev_done[kStreams - 1] = {};
+    char* scratch = nullptr;
+    size_t scratch_cap = 0;
+    DevMeta* h_meta = nullptr;
+    char* h_stage = nullptr;
+    size_t h_stage_cap = 0;
+};
+
 struct Context {
     bool ready = false;
     int device = -1;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // the context lane's main stream
     int cus = 0;
     int lds = 0;
     int clock_khz = 0;
@@ -79,15 +104,7 @@ struct Context {
     BinTable num{};
     int occ_sym_multi = 1, occ_sym_max = 1;
     int occ_num[2][kNumLdsVariants] = {};
-    static constexpr int kStreams = 4;  // fused-pass launches are spread over these
-    hipStream_t aux[kStreams - 1] = {};
-    hipEvent_t ev_ready = nullptr;
-    hipEvent_t ev_done[kStreams - 1] = {};
-    char* scratch = nullptr;
-    size_t scratch_cap = 0;
-    DevMeta* h_meta = nullptr;
-    char* h_stage = nullptr;
-    size_t h_stage_cap = 0;
+    Lane lane;
     bool profiling = false;
     std::vector<ProfRec> prof;
     std::vector<hipEvent_t> events;
@@ -159,6 +176,32 @@ spg_status check_num_table(const BinTable& t) {
 
 // ---------------------------------------------------------------- context
 
+spg_status lane_create(Lane& L) {
+    SPG_HIP(hipStreamCreateWithFlags(&L.s, hipStreamNonBlocking));
+    for (int i = 0; i < Lane::kStreams - 1; ++i) {
+        SPG_HIP(hipStreamCreateWithFlags(&L.aux[i], hipStreamNonBlocking));
+        SPG_HIP(hipEventCreateWithFlags(&L.ev_done[i], hipEventDisableTiming));
+    }
+    SPG_HIP(hipEventCreateWithFlags(&L.ev_ready, hipEventDisableTiming));
+    SPG_HIP(hipHostMalloc(reinterpret_cast<void**>(&L.h_meta), sizeof(DevMeta), hipHostMallocDefault));
+    return SPG_OK;
+}
+
+void lane_destroy(Lane& L) {
+    if (L.s) (void)hipStreamSynchronize(L.s);
+    for (int i = 0; i < Lane::kStreams - 1; ++i) {
+        if (L.aux[i]) (void)hipStreamSynchronize(L.aux[i]);
+        if (L.aux[i]) (void)hipStreamDestroy(L.aux[i]);
+        if (L.ev_done[i]) (void)hipEventDestroy(L.ev_done[i]);
+    }
+    if (L.ev_ready) (void)hipEventDestroy(L.ev_ready);
+    if (L.scratch) (void)hipFree(L.scratch);
+    if (L.h_meta) (void)hipHostFree(L.h_meta);
+    if (L.h_stage) (void)hipHostFree(L.h_stage);
+    if (L.s) (void)hipStreamDestroy(L.s);
+    L = Lane{};
+}
+
 spg_status ensure_context(Context*& out) {
     int dev = 0;
     SPG_HIP(hipGetDevice(&dev));
@@ -179,7 +222,6 @@ spg_status ensure_context(Context*& out) {
     if (c.lds < kNumBigLds + 1024 ||
         c.lds < kSymVariants[kSymMaxVariant].t * 4 + 8 * kSymVariants[kSymMaxVariant].ec + 1024)
         return fail(SPG_ERR_UNSUPPORTED_DEVICE, "device offers less LDS per workgroup than the kernels need");
-    SPG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     // Keep freed blocks in the stream-ordered pool: C is allocated per call.
     hipMemPool_t pool;
     if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
@@ -188,7 +230,8 @@ spg_status ensure_context(Context*& out) {
     } else {
         c.async_alloc = false;
     }
-    SPG_HIP(hipHostMalloc(reinterpret_cast<void**>(&c.h_meta), sizeof(DevMeta), hipHostMallocDefault));
+    SPG_TRY(lane_create(c.lane));
+    c.stream = c.lane.s;
     default_tables(c.sym, c.num);
     SPG_TRY(check_sym_table(c.sym));  // a stale tuned_tables.inc fails here, loudly
     SPG_TRY(check_num_table(c.num));
@@ -198,20 +241,15 @@ spg_status ensure_context(Context*& out) {
         c.occ_num[0][v] = occupancy_symnum_bin<float>(v);
         c.occ_num[1][v] = occupancy_symnum_bin<double>(v);
     }
-    for (int i = 0; i < Context::kStreams - 1; ++i) {
-        SPG_HIP(hipStreamCreateWithFlags(&c.aux[i], hipStreamNonBlocking));
-        SPG_HIP(hipEventCreateWithFlags(&c.ev_done[i], hipEventDisableTiming));
-    }
-    SPG_HIP(hipEventCreateWithFlags(&c.ev_ready, hipEventDisableTiming));
     c.ready = true;
     return SPG_OK;
 }
 
-spg_status dev_alloc(Context& c, void** p, size_t bytes) {
+spg_status dev_alloc(Context& c, void** p, size_t bytes, hipStream_t s) {
     *p = nullptr;
     if (bytes == 0) return SPG_OK;
     if (c.async_alloc) {
-        const hipError_t e = hipMallocAsync(p, bytes, c.stream);
+        const hipError_t e = hipMallocAsync(p, bytes, s);
         if (e == hipSuccess) return SPG_OK;
         if (e != hipErrorNotSupported) return hip_fail(e, "hipMallocAsync", __FILE__, __LINE__);
         c.async_alloc = false;
@@ -220,32 +258,33 @@ spg_status dev_alloc(Context& c, void** p, size_t bytes) {
     return SPG_OK;
 }
 
-spg_status dev_release(Context& c, void* p) {
+spg_status dev_release(Context& c, void* p, hipStream_t s) {
     if (!p) return SPG_OK;
-    if (c.async_alloc) SPG_HIP(hipFreeAsync(p, c.stream));
+    if (c.async_alloc) SPG_HIP(hipFreeAsync(p, s));
     else SPG_HIP(hipFree(p));
     return SPG_OK;
 }
 
-spg_status ensure_scratch(Context& c, size_t bytes) {
-    if (bytes <= c.scratch_cap) return SPG_OK;
-    SPG_HIP(hipStreamSynchronize(c.stream));
-    if (c.scratch) SPG_HIP(hipFree(c.scratch));
-    c.scratch = nullptr;
-    c.scratch_cap = 0;
-    const size_t cap = std::max(bytes, c.scratch_cap * 3 / 2);
-    SPG_HIP(hipMalloc(reinterpret_cast<void**>(&c.scratch), cap));
-    c.scratch_cap = cap;
+spg_status ensure_scratch(Lane& L, size_t bytes) {
+    if (bytes <= L.scratch_cap) return SPG_OK;
+    SPG_HIP(hipStreamSynchronize(L.s));
+    if (L.scratch) SPG_HIP(hipFree(L.scratch));
+    L.scratch = nullptr;
+    const size_t cap = std::max(bytes, L.scratch_cap * 3 / 2);
+    L.scratch_cap = 0;
+    SPG_HIP(hipMalloc(reinterpret_cast<void**>(&L.scratch), cap));
+    L.scratch_cap = cap;
     return SPG_OK;
 }
 
-spg_status ensure_stage(Context& c, size_t bytes) {
-    if (bytes <= c.h_stage_cap) return SPG_OK;
-    SPG_HIP(hipStreamSynchronize(c.stream));
-    if (c.h_stage) SPG_HIP(hipHostFree(c.h_stage));
-    c.h_stage = nullptr;
-    SPG_HIP(hipHostMalloc(reinterpret_cast<void**>(&c.h_stage), bytes, hipHostMallocDefault));
-    c.h_stage_cap = bytes;
+spg_status ensure_stage(Lane& L, size_t bytes) {
+    if (bytes <= L.h_stage_cap) return SPG_OK;
+    SPG_HIP(hipStreamSynchronize(L.s));
+    if (L.h_stage) SPG_HIP(hipHostFree(L.h_stage));
+    L.h_stage = nullptr;
+    L.h_stage_cap = 0;
+    SPG_HIP(hipHostMalloc(reinterpret_cast<void**>(&L.h_stage), bytes, hipHostMallocDefault));
+    L.h_stage_cap = bytes;
     return SPG_OK;
 }
 
@@ -253,8 +292,9 @@ spg_status ensure_stage(Context& c, size_t bytes) {
 
 struct Prof {
     Context& c;
+    bool on;
     int open = -1;
-    explicit Prof(Context& ctx) : c(ctx) {}
+    Prof(Context& ctx, bool enabled) : c(ctx), on(enabled) {}
     hipEvent_t next_event() {
         if (c.events_used == c.events.size()) {
             hipEvent_t e;
@@ -264,16 +304,16 @@ struct Prof {
         return c.events[c.events_used++];
     }
     hipStream_t st = nullptr;
-    void begin(const std::string& name, int64_t rows, int grid, hipStream_t on = nullptr) {
-        if (!c.profiling) return;
-        st = on ? on : c.stream;
+    void begin(const std::string& name, int64_t rows, int grid, hipStream_t where) {
+        if (!on) return;
+        st = where;
         ProfRec r{name, rows, grid, next_event(), next_event(), 0.0};
         if (r.ev0) (void)hipEventRecord(r.ev0, st);
         c.prof.push_back(r);
         open = static_cast<int>(c.prof.size()) - 1;
     }
     void end() {
-        if (!c.profiling || open < 0) return;
+        if (!on || open < 0) return;
         if (c.prof[open].ev1) (void)hipEventRecord(c.prof[open].ev1, st);
         open = -1;
     }
@@ -305,100 +345,209 @@ spg_status check_operands(const spg_csr& A, const spg_csr& B) {
     return SPG_OK;
 }
 
+// One batch of products on one lane, in two phases:
+//   symbolic()        NIP, binning, exact counts, scan (row pointers) and
+//                     the fused-pass bins; ends with one read-back, after
+//                     which nnz(C_i) of every product is known on the host
+//   launch_numeric()  allocates C.col / C.val of the exact sizes and queues
+//                     the fused pass on the lane's streams (no wait)
+//   finish()          waits for the lane and checks the consistency flag.
+// A plain call runs the three back to back; the server admits a batch by
+// output_bytes() between the first two.
+struct JobBase {
+    virtual ~JobBase() = default;
+    virtual spg_status symbolic(int n, const spg_csr* A, const spg_csr* B, spg_csr* out) = 0;
+    virtual spg_status launch_numeric() = 0;
+    virtual spg_status finish() = 0;
+    virtual void abort() = 0;
+    virtual size_t output_bytes() const = 0;
+};
+
 template <typename V>
-spg_status multiply_impl(Context& c, int count, const spg_csr* A, const spg_csr* B, spg_csr* C) {
-    const int di = sizeof(V) == 8 ? 1 : 0;
-    hipStream_t s = c.stream;
-    prof_reset(c);
-    Prof prof(c);
-    for (int i = 0; i < count; ++i) {
-        SPG_TRY(check_operands(A[i], B[i]));
-        C[i] = spg_csr{0, A[i].m, B[i].n, nullptr, nullptr, nullptr};
-    }
-    // output row pointers
+struct Job : JobBase {
+    Context& c;
+    Lane& L;
+    Prof prof;
+    int count = 0;
+    spg_csr* C = nullptr;
+    BinTable sym{}, num{};
     std::vector<void*> owned;
-    auto cleanup = [&]() {
-        for (void* p : owned) (void)dev_release(c, p);
-        for (int i = 0; i < count; ++i) C[i] = spg_csr{0, 0, 0, nullptr, nullptr, nullptr};
-    };
+    std::vector<Prob<V>> probs;
+    std::vector<Tile> tiles;
+    ProbSet<V> ps{};
     long long total_rows = 0;
     int n_tiles = 0;
-    for (int i = 0; i < count; ++i) {
-        void* p = nullptr;
-        const spg_status st = dev_alloc(c, &p, sizeof(int32_t) * (static_cast<size_t>(A[i].m) + 1));
-        if (st != SPG_OK) {
-            cleanup();
-            return st;
-        }
-        owned.push_back(p);
-        C[i].rpt = static_cast<int32_t*>(p);
-        total_rows += A[i].m;
-        n_tiles += (A[i].m + kTileRows - 1) / kTileRows;
-        if (A[i].m == 0) {
-            const hipError_t e = hipMemsetAsync(C[i].rpt, 0, sizeof(int32_t), s);
-            if (e != hipSuccess) {
-                cleanup();
-                return hip_fail(e, "hipMemsetAsync", __FILE__, __LINE__);
-            }
-        }
+    bool batch = false;
+    bool empty = false;
+    DevMeta* meta = nullptr;
+    long long* tile_sums = nullptr;
+    long long* inst_nnz = nullptr;
+    int2* items1 = nullptr;
+    int2* items2 = nullptr;
+    int2* fail_items = nullptr;
+    long long* fail_off = nullptr;
+    long long* gpool_off = nullptr;
+    Prob<V>* d_probs = nullptr;
+    Tile* d_tiles = nullptr;
+    size_t stage_probs = 0, stage_tiles = 0, stage_inst = 0;
+    // counters read back after the symbolic phase (DevMeta itself is
+    // over-aligned; a Job is deleted through its base)
+    struct {
+        int32_t flags = 0;
+        int32_t num_count[kMaxBins] = {};
+        unsigned long long gnum_pool = 0;
+        long long nnz_total = 0;
+    } hm;
+    std::vector<long long> nnz;
+    void* num_pool = nullptr;
+
+    Job(Context& ctx, Lane& lane, const BinTable& sym_table, const BinTable& num_table, bool profile)
+        : c(ctx), L(lane), prof(ctx, profile), sym(sym_table), num(num_table) {}
+
+    void abort() override {
+        for (void* p : owned) (void)dev_release(c, p, L.s);
+        owned.clear();
+        if (num_pool) (void)dev_release(c, num_pool, L.s);
+        num_pool = nullptr;
+        for (int i = 0; i < count; ++i) C[i] = spg_csr{0, 0, 0, nullptr, nullptr, nullptr};
     }
-    if (total_rows >= INT_MAX) {
-        cleanup();
-        return fail(SPG_ERR_OVERFLOW, "total row count exceeds int32");
+
+    spg_status failed(spg_status st) {
+        abort();
+        return st;
     }
-    if (n_tiles == 0) {  // every product has zero rows
-        const hipError_t e = hipStreamSynchronize(s);
-        if (e != hipSuccess) {
-            cleanup();
-            return hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__);
+
+    size_t output_bytes() const override {
+        size_t b = 0;
+        for (long long n : nnz) b += static_cast<size_t>(n) * (sizeof(int32_t) + sizeof(V));
+        return b;
+    }
+
+    spg_status front() {
+        const hipStream_t s = L.s;
+        if (batch) {
+            SPG_HIP(hipMemcpyAsync(d_probs, L.h_stage + stage_probs, sizeof(Prob<V>) * count, hipMemcpyHostToDevice,
+                                   s));
+            SPG_HIP(hipMemcpyAsync(d_tiles, L.h_stage + stage_tiles, sizeof(Tile) * n_tiles, hipMemcpyHostToDevice,
+                                   s));
         }
+        prof.begin("clear", 0, 1, s);
+        launch_clear(meta, 0, s);
+        prof.end();
+        prof.begin("nip", total_rows, n_tiles, s);
+        launch_nip<V>(ps, sym, meta, s);
+        prof.end();
+        prof.begin("scatter_sym", total_rows, n_tiles, s);
+        launch_scatter<V>(ps, sym, meta, items1, s);
+        prof.end();
+        // The largest-table bin runs on an aux stream beside the multi-bin
+        // launch (it is queued first: its workgroups without a row exit at
+        // once and leave the CUs to the multi-bin grid).
+        SPG_HIP(hipEventRecord(L.ev_ready, s));
+        SPG_HIP(hipStreamWaitEvent(L.aux[0], L.ev_ready, 0));
+        {
+            const int grid = c.cus * c.occ_sym_max;
+            prof.begin("sym_max", 0, grid, L.aux[0]);
+            launch_sym_max<V>(ps, items1, sym, sym.n - 1, meta, fail_items, fail_off, grid, L.aux[0]);
+            prof.end();
+        }
+        const int n_multi = sym.n - 1;
+        if (n_multi > 0) {
+            int grid = c.cus * c.occ_sym_multi;
+            grid = (grid + kXcds - 1) / kXcds * kXcds;
+            prof.begin("sym_multi", total_rows, grid, s);
+            launch_sym_multi<V>(ps, items1, sym, n_multi, meta, grid, s);
+            prof.end();
+        }
+        SPG_HIP(hipEventRecord(L.ev_done[0], L.aux[0]));
+        SPG_HIP(hipStreamWaitEvent(s, L.ev_done[0], 0));
         return SPG_OK;
     }
-    const size_t M = static_cast<size_t>(total_rows);
-    const bool batch = count > 1;
 
-    // scratch layout
-    size_t off = 0;
-    auto take = [&](size_t bytes) {
-        const size_t at = off;
-        off = align256(off + bytes);
-        return at;
-    };
-    const size_t o_meta = take(sizeof(DevMeta));
-    const size_t o_tsum = take(sizeof(long long) * n_tiles);
-    const size_t o_inst = take(sizeof(long long) * count);
-    const size_t o_nip = take(sizeof(int32_t) * M);
-    const size_t o_it1 = take(sizeof(int2) * M);
-    const size_t o_it2 = take(sizeof(int2) * M);
-    const size_t o_fit = take(sizeof(int2) * M);
-    const size_t o_foff = take(sizeof(long long) * M);
-    const size_t o_goff = take(sizeof(long long) * M);
-    const size_t o_probs = take(batch ? sizeof(Prob<V>) * count : 0);
-    const size_t o_tiles = take(batch ? sizeof(Tile) * n_tiles : 0);
-    {
-        const spg_status st = ensure_scratch(c, off);
-        if (st != SPG_OK) {
-            cleanup();
-            return st;
-        }
+    spg_status scan(int after_fail) {
+        const hipStream_t s = L.s;
+        prof.begin("scan_count", total_rows, n_tiles, s);
+        launch_scan_count<V>(ps, num, meta, tile_sums, after_fail, s);
+        prof.end();
+        prof.begin("scan_apply", total_rows, n_tiles, s);
+        launch_scan_apply<V>(ps, num, meta, tile_sums, items2, gpool_off, inst_nnz, after_fail, s);
+        prof.end();
+        SPG_HIP(hipMemcpyAsync(L.h_meta, meta, sizeof(DevMeta), hipMemcpyDeviceToHost, s));
+        if (batch)
+            SPG_HIP(hipMemcpyAsync(L.h_stage + stage_inst, inst_nnz, sizeof(long long) * count, hipMemcpyDeviceToHost,
+                                   s));
+        SPG_HIP(hipStreamSynchronize(s));
+        return SPG_OK;
     }
-    char* base = c.scratch;
-    DevMeta* meta = reinterpret_cast<DevMeta*>(base + o_meta);
-    long long* tile_sums = reinterpret_cast<long long*>(base + o_tsum);
-    long long* inst_nnz = reinterpret_cast<long long*>(base + o_inst);
-    int32_t* nip = reinterpret_cast<int32_t*>(base + o_nip);
-    int2* items1 = reinterpret_cast<int2*>(base + o_it1);
-    int2* items2 = reinterpret_cast<int2*>(base + o_it2);
-    int2* fail_items = reinterpret_cast<int2*>(base + o_fit);
-    long long* fail_off = reinterpret_cast<long long*>(base + o_foff);
-    long long* gpool_off = reinterpret_cast<long long*>(base + o_goff);
-    Prob<V>* d_probs = reinterpret_cast<Prob<V>*>(base + o_probs);
-    Tile* d_tiles = reinterpret_cast<Tile*>(base + o_tiles);
 
-    // descriptors
-    std::vector<Prob<V>> probs(count);
-    std::vector<Tile> tiles;
-    {
+    spg_status symbolic(int n, const spg_csr* A, const spg_csr* B, spg_csr* out) override {
+        count = n;
+        C = out;
+        batch = count > 1;
+        const hipStream_t s = L.s;
+        for (int i = 0; i < count; ++i) {
+            SPG_TRY(check_operands(A[i], B[i]));
+            C[i] = spg_csr{0, A[i].m, B[i].n, nullptr, nullptr, nullptr};
+        }
+        // output row pointers
+        for (int i = 0; i < count; ++i) {
+            void* p = nullptr;
+            const spg_status st = dev_alloc(c, &p, sizeof(int32_t) * (static_cast<size_t>(A[i].m) + 1), s);
+            if (st != SPG_OK) return failed(st);
+            owned.push_back(p);
+            C[i].rpt = static_cast<int32_t*>(p);
+            total_rows += A[i].m;
+            n_tiles += (A[i].m + kTileRows - 1) / kTileRows;
+            if (A[i].m == 0) {
+                const hipError_t e = hipMemsetAsync(C[i].rpt, 0, sizeof(int32_t), s);
+                if (e != hipSuccess) return failed(hip_fail(e, "hipMemsetAsync", __FILE__, __LINE__));
+            }
+        }
+        nnz.assign(count, 0);
+        if (total_rows >= INT_MAX) return failed(fail(SPG_ERR_OVERFLOW, "total row count exceeds int32"));
+        if (n_tiles == 0) {  // every product has zero rows
+            empty = true;
+            return SPG_OK;
+        }
+        const size_t M = static_cast<size_t>(total_rows);
+
+        // scratch layout
+        size_t off = 0;
+        auto take = [&](size_t bytes) {
+            const size_t at = off;
+            off = align256(off + bytes);
+            return at;
+        };
+        const size_t o_meta = take(sizeof(DevMeta));
+        const size_t o_tsum = take(sizeof(long long) * n_tiles);
+        const size_t o_inst = take(sizeof(long long) * count);
+        const size_t o_nip = take(sizeof(int32_t) * M);
+        const size_t o_it1 = take(sizeof(int2) * M);
+        const size_t o_it2 = take(sizeof(int2) * M);
+        const size_t o_fit = take(sizeof(int2) * M);
+        const size_t o_foff = take(sizeof(long long) * M);
+        const size_t o_goff = take(sizeof(long long) * M);
+        const size_t o_probs = take(batch ? sizeof(Prob<V>) * count : 0);
+        const size_t o_tiles = take(batch ? sizeof(Tile) * n_tiles : 0);
+        {
+            const spg_status st = ensure_scratch(L, off);
+            if (st != SPG_OK) return failed(st);
+        }
+        char* base = L.scratch;
+        meta = reinterpret_cast<DevMeta*>(base + o_meta);
+        tile_sums = reinterpret_cast<long long*>(base + o_tsum);
+        inst_nnz = reinterpret_cast<long long*>(base + o_inst);
+        int32_t* nip = reinterpret_cast<int32_t*>(base + o_nip);
+        items1 = reinterpret_cast<int2*>(base + o_it1);
+        items2 = reinterpret_cast<int2*>(base + o_it2);
+        fail_items = reinterpret_cast<int2*>(base + o_fit);
+        fail_off = reinterpret_cast<long long*>(base + o_foff);
+        gpool_off = reinterpret_cast<long long*>(base + o_goff);
+        d_probs = reinterpret_cast<Prob<V>*>(base + o_probs);
+        d_tiles = reinterpret_cast<Tile*>(base + o_tiles);
+
+        // descriptors
+        probs.assign(count, Prob<V>{});
         long long row_base = 0;
         for (int i = 0; i < count; ++i) {
             Prob<V>& p = probs[i];
@@ -421,230 +570,335 @@ spg_status multiply_impl(Context& c, int count, const spg_csr* A, const spg_csr*
             }
             row_base += A[i].m;
         }
-    }
-    ProbSet<V> ps{};
-    ps.p0 = probs[0];
-    ps.n_inst = count;
-    ps.n_tiles = n_tiles;
-    ps.many = batch ? d_probs : nullptr;
-    ps.tiles = batch ? d_tiles : nullptr;
-    size_t stage_probs = 0, stage_tiles = 0, stage_inst = 0;
-    if (batch) {
-        stage_probs = 0;
-        stage_tiles = align256(sizeof(Prob<V>) * count);
-        stage_inst = stage_tiles + align256(sizeof(Tile) * n_tiles);
-        const spg_status st = ensure_stage(c, stage_inst + align256(sizeof(long long) * count));
-        if (st != SPG_OK) {
-            cleanup();
-            return st;
-        }
-        std::memcpy(c.h_stage + stage_probs, probs.data(), sizeof(Prob<V>) * count);
-        std::memcpy(c.h_stage + stage_tiles, tiles.data(), sizeof(Tile) * n_tiles);
-    }
-
-    auto run_front = [&]() -> spg_status {
+        ps.p0 = probs[0];
+        ps.n_inst = count;
+        ps.n_tiles = n_tiles;
+        ps.many = batch ? d_probs : nullptr;
+        ps.tiles = batch ? d_tiles : nullptr;
         if (batch) {
-            SPG_HIP(hipMemcpyAsync(d_probs, c.h_stage + stage_probs, sizeof(Prob<V>) * count, hipMemcpyHostToDevice,
-                                   s));
-            SPG_HIP(hipMemcpyAsync(d_tiles, c.h_stage + stage_tiles, sizeof(Tile) * n_tiles, hipMemcpyHostToDevice,
-                                   s));
+            stage_probs = 0;
+            stage_tiles = align256(sizeof(Prob<V>) * count);
+            stage_inst = stage_tiles + align256(sizeof(Tile) * n_tiles);
+            const spg_status st = ensure_stage(L, stage_inst + align256(sizeof(long long) * count));
+            if (st != SPG_OK) return failed(st);
+            std::memcpy(L.h_stage + stage_probs, probs.data(), sizeof(Prob<V>) * count);
+            std::memcpy(L.h_stage + stage_tiles, tiles.data(), sizeof(Tile) * n_tiles);
         }
-        prof.begin("clear", 0, 1);
-        launch_clear(meta, 0, s);
-        prof.end();
-        prof.begin("nip", total_rows, n_tiles);
-        launch_nip<V>(ps, c.sym, meta, s);
-        prof.end();
-        prof.begin("scatter_sym", total_rows, n_tiles);
-        launch_scatter<V>(ps, c.sym, meta, items1, s);
-        prof.end();
-        // The largest-table bin runs on an aux stream beside the multi-bin
-        // launch (it is queued first: its workgroups without a row exit at
-        // once and leave the CUs to the multi-bin grid).
-        SPG_HIP(hipEventRecord(c.ev_ready, s));
-        SPG_HIP(hipStreamWaitEvent(c.aux[0], c.ev_ready, 0));
-        {
-            const int grid = c.cus * c.occ_sym_max;
-            prof.begin("sym_max", 0, grid, c.aux[0]);
-            launch_sym_max<V>(ps, items1, c.sym, c.sym.n - 1, meta, fail_items, fail_off, grid, c.aux[0]);
-            prof.end();
-        }
-        const int n_multi = c.sym.n - 1;
-        if (n_multi > 0) {
-            int grid = c.cus * c.occ_sym_multi;
-            grid = (grid + kXcds - 1) / kXcds * kXcds;
-            prof.begin("sym_multi", total_rows, grid);
-            launch_sym_multi<V>(ps, items1, c.sym, n_multi, meta, grid, s);
-            prof.end();
-        }
-        SPG_HIP(hipEventRecord(c.ev_done[0], c.aux[0]));
-        SPG_HIP(hipStreamWaitEvent(s, c.ev_done[0], 0));
-        return SPG_OK;
-    };
-    auto run_scan = [&](int after_fail) -> spg_status {
-        prof.begin("scan_count", total_rows, n_tiles);
-        launch_scan_count<V>(ps, c.num, meta, tile_sums, after_fail, s);
-        prof.end();
-        prof.begin("scan_apply", total_rows, n_tiles);
-        launch_scan_apply<V>(ps, c.num, meta, tile_sums, items2, gpool_off, inst_nnz, after_fail, s);
-        prof.end();
-        SPG_HIP(hipMemcpyAsync(c.h_meta, meta, sizeof(DevMeta), hipMemcpyDeviceToHost, s));
-        if (batch)
-            SPG_HIP(hipMemcpyAsync(c.h_stage + stage_inst, inst_nnz, sizeof(long long) * count,
-                                   hipMemcpyDeviceToHost, s));
-        SPG_HIP(hipStreamSynchronize(s));
-        return SPG_OK;
-    };
 
-    spg_status st = run_front();
-    if (st == SPG_OK) st = run_scan(0);
-    void* sym_pool = nullptr;
-    if (st == SPG_OK && c.h_meta->fail_count.v > 0) {
-        // rows that overflowed the largest LDS table: count them again in
-        // global memory, then redo the scan stage
-        const int fails = c.h_meta->fail_count.v;
-        st = dev_alloc(c, &sym_pool, sizeof(int32_t) * c.h_meta->fail_pool.v);
-        if (st == SPG_OK) {
-            prof.begin("sym_global", fails, fails);
-            launch_sym_global<V>(ps, fail_items, fail_off, static_cast<int*>(sym_pool), fails, s);
-            prof.end();
-            launch_clear(meta, kMetaScanWord, s);
-            st = run_scan(1);
-        }
-        if (sym_pool) (void)dev_release(c, sym_pool);
-    }
-    if (st != SPG_OK) {
-        cleanup();
-        return st;
-    }
-    const DevMeta hm = *c.h_meta;
-    if (hm.flags.v & kFlagNipOverflow) {
-        cleanup();
-        return fail(SPG_ERR_OVERFLOW, "a row has 2^31 or more intermediate products");
-    }
-    if (hm.flags.v & kFlagNnzOverflow) {
-        cleanup();
-        return fail(SPG_ERR_OVERFLOW, "nnz(C) does not fit int32");
-    }
-
-    // exact-size outputs
-    for (int i = 0; i < count; ++i) {
-        const long long nnz = batch ? reinterpret_cast<long long*>(c.h_stage + stage_inst)[i]
-                                    : (A[i].m > 0 ? hm.nnz_total.v : 0);
-        if (A[i].m == 0) continue;
-        if (nnz >= INT_MAX) {
-            cleanup();
-            return fail(SPG_ERR_OVERFLOW, "nnz(C) does not fit int32");
-        }
-        C[i].nnz = static_cast<int32_t>(nnz);
-        void *pc = nullptr, *pv = nullptr;
-        st = dev_alloc(c, &pc, sizeof(int32_t) * nnz);
-        if (st == SPG_OK) owned.push_back(pc);
-        if (st == SPG_OK) st = dev_alloc(c, &pv, sizeof(V) * nnz);
-        if (st == SPG_OK) owned.push_back(pv);
-        if (st != SPG_OK) {
-            cleanup();
-            return st;
-        }
-        C[i].col = static_cast<int32_t*>(pc);
-        C[i].val = pv;
-        probs[i].c_col = C[i].col;
-        probs[i].c_val = static_cast<V*>(pv);
-    }
-    ps.p0 = probs[0];
-    if (batch) {
-        std::memcpy(c.h_stage + stage_probs, probs.data(), sizeof(Prob<V>) * count);
-        const hipError_t e =
-            hipMemcpyAsync(d_probs, c.h_stage + stage_probs, sizeof(Prob<V>) * count, hipMemcpyHostToDevice, s);
-        if (e != hipSuccess) {
-            cleanup();
-            return hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__);
-        }
-    }
-
-    // fused pass: one launch per non-empty bin, heaviest bins first, spread
-    // over the context's streams so that small bins overlap
-    struct Launch {
-        int variant, off, count;
-    };
-    std::vector<Launch> launches;
-    int glob_bin = -1, glob_off = 0;
-    {
-        int item_off = 0;
-        for (int b = 0; b < c.num.n; ++b) {
-            const int cnt = hm.num_count[b].v;
-            const int v = c.num.variant[b];
-            if (cnt > 0) {
-                if (v == kNumGlobalVariant) {
-                    glob_bin = b;
-                    glob_off = item_off;
-                } else {
-                    launches.push_back(Launch{v, item_off, cnt});
-                }
+        spg_status st = front();
+        if (st == SPG_OK) st = scan(0);
+        if (st == SPG_OK && L.h_meta->fail_count.v > 0) {
+            // rows that overflowed the largest LDS table: count them again in
+            // global memory, then redo the scan stage
+            const int fails = L.h_meta->fail_count.v;
+            void* sym_pool = nullptr;
+            st = dev_alloc(c, &sym_pool, sizeof(int32_t) * L.h_meta->fail_pool.v, s);
+            if (st == SPG_OK) {
+                prof.begin("sym_global", fails, fails, s);
+                launch_sym_global<V>(ps, fail_items, fail_off, static_cast<int*>(sym_pool), fails, s);
+                prof.end();
+                launch_clear(meta, kMetaScanWord, s);
+                st = scan(1);
             }
-            item_off += cnt;
+            if (sym_pool) (void)dev_release(c, sym_pool, s);
         }
-    }
-    std::stable_sort(launches.begin(), launches.end(), [](const Launch& x, const Launch& y) {
-        return kNumVariants[x.variant].p * kNumVariants[x.variant].wg / kNumVariants[x.variant].r >
-               kNumVariants[y.variant].p * kNumVariants[y.variant].wg / kNumVariants[y.variant].r;
-    });
-    const int n_launch = static_cast<int>(launches.size()) + (glob_bin >= 0 ? 1 : 0);
-    const int n_streams = std::min(n_launch, static_cast<int>(Context::kStreams));
-    auto stream_of = [&](int k) { return k % Context::kStreams == 0 ? s : c.aux[k % Context::kStreams - 1]; };
-    void* num_pool = nullptr;
-    if (glob_bin >= 0) {
-        st = dev_alloc(c, &num_pool, sizeof(int32_t) * hm.gnum_pool.v);
-        if (st != SPG_OK) {
-            cleanup();
-            return st;
+        if (st != SPG_OK) return failed(st);
+        hm.flags = L.h_meta->flags.v;
+        for (int b = 0; b < kMaxBins; ++b) hm.num_count[b] = L.h_meta->num_count[b].v;
+        hm.gnum_pool = L.h_meta->gnum_pool.v;
+        hm.nnz_total = L.h_meta->nnz_total.v;
+        if (hm.flags & kFlagNipOverflow)
+            return failed(fail(SPG_ERR_OVERFLOW, "a row has 2^31 or more intermediate products"));
+        if (hm.flags & kFlagNnzOverflow) return failed(fail(SPG_ERR_OVERFLOW, "nnz(C) does not fit int32"));
+        for (int i = 0; i < count; ++i) {
+            if (A[i].m == 0) continue;
+            nnz[i] = batch ? reinterpret_cast<long long*>(L.h_stage + stage_inst)[i] : hm.nnz_total;
+            if (nnz[i] >= INT_MAX) return failed(fail(SPG_ERR_OVERFLOW, "nnz(C) does not fit int32"));
         }
+        return SPG_OK;
     }
-    hipError_t e = hipSuccess;
-    if (n_streams > 1) {  // C.col, C.val and the pool were allocated on the main stream
-        e = hipEventRecord(c.ev_ready, s);
-        for (int i = 1; i < n_streams && e == hipSuccess; ++i) e = hipStreamWaitEvent(c.aux[i - 1], c.ev_ready, 0);
+
+    spg_status launch_numeric() override {
+        if (empty) return SPG_OK;
+        const hipStream_t s = L.s;
+        const int di = sizeof(V) == 8 ? 1 : 0;
+        // exact-size outputs
+        for (int i = 0; i < count; ++i) {
+            if (C[i].m == 0) continue;
+            C[i].nnz = static_cast<int32_t>(nnz[i]);
+            void *pc = nullptr, *pv = nullptr;
+            spg_status st = dev_alloc(c, &pc, sizeof(int32_t) * nnz[i], s);
+            if (st == SPG_OK) owned.push_back(pc);
+            if (st == SPG_OK) st = dev_alloc(c, &pv, sizeof(V) * nnz[i], s);
+            if (st == SPG_OK) owned.push_back(pv);
+            if (st != SPG_OK) return failed(st);
+            C[i].col = static_cast<int32_t*>(pc);
+            C[i].val = pv;
+            probs[i].c_col = C[i].col;
+            probs[i].c_val = static_cast<V*>(pv);
+        }
+        ps.p0 = probs[0];
+        if (batch) {
+            std::memcpy(L.h_stage + stage_probs, probs.data(), sizeof(Prob<V>) * count);
+            const hipError_t e =
+                hipMemcpyAsync(d_probs, L.h_stage + stage_probs, sizeof(Prob<V>) * count, hipMemcpyHostToDevice, s);
+            if (e != hipSuccess) return failed(hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__));
+        }
+
+        // fused pass: one launch per non-empty bin, heaviest bins first,
+        // spread over the lane's streams so that small bins overlap
+        struct Launch {
+            int variant, off, count;
+        };
+        std::vector<Launch> launches;
+        int glob_bin = -1, glob_off = 0;
+        {
+            int item_off = 0;
+            for (int b = 0; b < num.n; ++b) {
+                const int cnt = hm.num_count[b];
+                const int v = num.variant[b];
+                if (cnt > 0) {
+                    if (v == kNumGlobalVariant) {
+                        glob_bin = b;
+                        glob_off = item_off;
+                    } else {
+                        launches.push_back(Launch{v, item_off, cnt});
+                    }
+                }
+                item_off += cnt;
+            }
+        }
+        std::stable_sort(launches.begin(), launches.end(), [](const Launch& x, const Launch& y) {
+            return kNumVariants[x.variant].p * kNumVariants[x.variant].wg / kNumVariants[x.variant].r >
+                   kNumVariants[y.variant].p * kNumVariants[y.variant].wg / kNumVariants[y.variant].r;
+        });
+        const int n_launch = static_cast<int>(launches.size()) + (glob_bin >= 0 ? 1 : 0);
+        const int n_streams = std::min(n_launch, static_cast<int>(Lane::kStreams));
+        auto stream_of = [&](int k) { return k % Lane::kStreams == 0 ? s : L.aux[k % Lane::kStreams - 1]; };
+        if (glob_bin >= 0) {
+            const spg_status st = dev_alloc(c, &num_pool, sizeof(int32_t) * hm.gnum_pool, s);
+            if (st != SPG_OK) return failed(st);
+        }
+        hipError_t e = hipSuccess;
+        if (n_streams > 1) {  // C.col, C.val and the pool were allocated on the main stream
+            e = hipEventRecord(L.ev_ready, s);
+            for (int i = 1; i < n_streams && e == hipSuccess; ++i) e = hipStreamWaitEvent(L.aux[i - 1], L.ev_ready, 0);
+        }
+        int k = 0;
+        if (e == hipSuccess && glob_bin >= 0) {
+            const hipStream_t on = stream_of(k++);
+            prof.begin("symnum_global", hm.num_count[glob_bin], hm.num_count[glob_bin], on);
+            launch_symnum_global<V>(ps, items2, gpool_off, glob_off, hm.num_count[glob_bin],
+                                    static_cast<int*>(num_pool), meta, on);
+            prof.end();
+        }
+        for (const Launch& Lh : launches) {
+            if (e != hipSuccess) break;
+            const NumVariant& nv = kNumVariants[Lh.variant];
+            const int units = (Lh.count + nv.r - 1) / nv.r;
+            int grid = std::min(units, c.cus * c.occ_num[di][Lh.variant]);
+            grid = (grid + kXcds - 1) / kXcds * kXcds;
+            const hipStream_t on = stream_of(k++);
+            prof.begin("symnum_v" + std::to_string(Lh.variant), Lh.count, grid, on);
+            e = launch_symnum_bin<V>(Lh.variant, ps, items2, Lh.off, Lh.count, meta, grid, on);
+            prof.end();
+        }
+        for (int i = 1; i < n_streams && e == hipSuccess; ++i) {
+            e = hipEventRecord(L.ev_done[i - 1], L.aux[i - 1]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s, L.ev_done[i - 1], 0);
+        }
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(&L.h_meta->flags.v, &meta->flags.v, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+        if (num_pool && e == hipSuccess) {
+            (void)dev_release(c, num_pool, s);
+            num_pool = nullptr;
+        }
+        if (e != hipSuccess) return failed(hip_fail(e, "fused pass", __FILE__, __LINE__));
+        return SPG_OK;
     }
-    int k = 0;
-    if (e == hipSuccess && glob_bin >= 0) {
-        const hipStream_t on = stream_of(k++);
-        prof.begin("symnum_global", hm.num_count[glob_bin].v, hm.num_count[glob_bin].v, on);
-        launch_symnum_global<V>(ps, items2, gpool_off, glob_off, hm.num_count[glob_bin].v,
-                                static_cast<int*>(num_pool), meta, on);
-        prof.end();
+
+    spg_status finish() override {
+        const hipError_t e = hipStreamSynchronize(L.s);
+        if (e != hipSuccess) return failed(hip_fail(e, "fused pass", __FILE__, __LINE__));
+        if (!empty && (L.h_meta->flags.v & kFlagMismatch))
+            return failed(fail(SPG_ERR_INTERNAL, "fused pass found a different column count than the count pass "
+                                                 "(are B's rows free of duplicate columns and indices in range?)"));
+        owned.clear();  // the outputs now belong to the caller
+        return SPG_OK;
     }
-    for (const Launch& L : launches) {
-        if (e != hipSuccess) break;
-        const NumVariant& nv = kNumVariants[L.variant];
-        const int units = (L.count + nv.r - 1) / nv.r;
-        int grid = std::min(units, c.cus * c.occ_num[di][L.variant]);
-        grid = (grid + kXcds - 1) / kXcds * kXcds;
-        const hipStream_t on = stream_of(k++);
-        prof.begin("symnum_v" + std::to_string(L.variant), L.count, grid, on);
-        e = launch_symnum_bin<V>(L.variant, ps, items2, L.off, L.count, meta, grid, on);
-        prof.end();
-    }
-    for (int i = 1; i < n_streams && e == hipSuccess; ++i) {
-        e = hipEventRecord(c.ev_done[i - 1], c.aux[i - 1]);
-        if (e == hipSuccess) e = hipStreamWaitEvent(s, c.ev_done[i - 1], 0);
-    }
-    if (e == hipSuccess) e = hipGetLastError();
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(&c.h_meta->flags.v, &meta->flags.v, sizeof(int32_t), hipMemcpyDeviceToHost, s);
-    if (num_pool) (void)dev_release(c, num_pool);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) {
-        cleanup();
-        return hip_fail(e, "fused pass", __FILE__, __LINE__);
-    }
-    if (c.h_meta->flags.v & kFlagMismatch) {
-        cleanup();
-        return fail(SPG_ERR_INTERNAL, "fused pass found a different column count than the count pass "
-                                      "(are B's rows free of duplicate columns and indices in range?)");
-    }
+};
+
+template <typename V>
+spg_status multiply_impl(Context& c, int count, const spg_csr* A, const spg_csr* B, spg_csr* C) {
+    prof_reset(c);
+    Job<V> job(c, c.lane, c.sym, c.num, c.profiling);
+    SPG_TRY(job.symbolic(count, A, B, C));
+    SPG_TRY(job.launch_numeric());
+    SPG_TRY(job.finish());
     if (c.profiling) prof_collect(c);
     return SPG_OK;
 }
+
+// ---------------------------------------------------------------- server
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+    int64_t ticket = 0;
+    spg_csr A{}, B{};
+    spg_dtype dtype = SPG_F64;
+    spg_csr C{};
+    spg_status st = SPG_OK;
+    std::string err;
+    double t_submit = 0, t_done = 0;
+    bool done = false;
+};
+
+// One admitted batch on one lane.  C stays in place while the job runs.
+struct Flight {
+    int lane = -1;
+    std::unique_ptr<JobBase> job;
+    std::vector<std::shared_ptr<Request>> reqs;
+    std::vector<spg_csr> A, B, C;
+    size_t bytes = 0;
+};
+
+struct Server {
+    static constexpr int kLanes = 3;
+    // A batch gathers requests up to this many nonzeros of A in total; a
+    // larger request runs alone, so it never holds small ones back.
+    static constexpr long long kBatchNnz = 4 << 20;
+    Context* c = nullptr;
+    size_t budget = 0;
+    int max_batch = 1;
+    Lane lanes[kLanes];
+    std::mutex mu;
+    std::condition_variable cv_work;  // requests queued / stop
+    std::condition_variable cv_done;  // a request finished
+    std::condition_variable cv_mem;   // an admitted batch released its output budget
+    std::deque<std::shared_ptr<Request>> pending;
+    std::unordered_map<int64_t, std::shared_ptr<Request>> requests;
+    int64_t next_ticket = 1;
+    bool stop = false;
+    size_t in_flight = 0;  // output bytes of admitted batches still running
+    int admitted = 0;      // such batches
+    std::thread workers[kLanes];
+    int64_t n_batches = 0, n_served = 0, max_seen = 0, n_deferred = 0;
+    size_t peak_bytes = 0;  // largest in_flight seen
+
+    void publish(Flight& f, spg_status st, const std::string& err) {
+        const double t = now_s();
+        std::lock_guard<std::mutex> lk(mu);
+        for (size_t i = 0; i < f.reqs.size(); ++i) {
+            Request& r = *f.reqs[i];
+            r.st = st;
+            r.err = err;
+            if (st == SPG_OK) r.C = f.C[i];
+            r.t_done = t;
+            r.done = true;
+        }
+        n_served += static_cast<int64_t>(f.reqs.size());
+        cv_done.notify_all();
+    }
+
+    // One worker per lane: take a batch, run its symbolic phase, wait until
+    // its exact output fits the budget beside the batches the other lanes
+    // are running, run the fused pass, publish.  Lanes work concurrently, so
+    // a small batch never waits behind the symbolic phase of a large one.
+    void run(int lane) {
+        (void)hipSetDevice(c->device);
+        Lane& L = lanes[lane];
+        int cap = max_batch;
+        while (true) {
+            std::vector<std::shared_ptr<Request>> take;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] { return stop || !pending.empty(); });
+                if (pending.empty()) return;  // stop, and the queue is drained
+                // the oldest request and the next ones of its value type, up
+                // to `cap` requests and kBatchNnz nonzeros of A
+                const spg_dtype dt = pending.front()->dtype;
+                long long nz = 0;
+                for (auto it = pending.begin(); it != pending.end() && static_cast<int>(take.size()) < cap;) {
+                    const long long w = (*it)->A.nnz;
+                    if ((*it)->dtype == dt && (take.empty() || nz + w <= kBatchNnz)) {
+                        nz += w;
+                        take.push_back(*it);
+                        it = pending.erase(it);
+                    } else {
+                        ++it;
+                    }
+                }
+            }
+            Flight f;
+            f.lane = lane;
+            f.reqs = take;
+            const int n = static_cast<int>(take.size());
+            for (auto& r : take) {
+                f.A.push_back(r->A);
+                f.B.push_back(r->B);
+            }
+            f.C.assign(n, spg_csr{});
+            BinTable sym, num;
+            {
+                std::lock_guard<std::recursive_mutex> g(g_mutex);
+                sym = c->sym;
+                num = c->num;
+            }
+            if (take.front()->dtype == SPG_F64)
+                f.job = std::make_unique<Job<double>>(*c, L, sym, num, false);
+            else
+                f.job = std::make_unique<Job<float>>(*c, L, sym, num, false);
+            spg_status st = f.job->symbolic(n, f.A.data(), f.B.data(), f.C.data());
+            if (st != SPG_OK) {
+                publish(f, st, g_last_error);
+                continue;
+            }
+            f.bytes = f.job->output_bytes();
+            bool admit = false;
+            {
+                // admission by exact output memory
+                std::unique_lock<std::mutex> lk(mu);
+                cv_mem.wait(lk, [&] { return in_flight + f.bytes <= budget || admitted == 0; });
+                if (in_flight + f.bytes <= budget) {
+                    admit = true;
+                    in_flight += f.bytes;
+                    peak_bytes = std::max(peak_bytes, in_flight);
+                    ++admitted;
+                    ++n_batches;
+                    max_seen = std::max<int64_t>(max_seen, n);
+                } else if (n > 1) {  // over budget even alone: split, back to the front of the queue
+                    for (int i = n - 1; i >= 0; --i) pending.push_front(take[i]);
+                    n_deferred += n;
+                }
+            }
+            if (!admit) {
+                f.job->abort();
+                if (n > 1) {
+                    cap = std::max(1, n / 2);
+                    cv_work.notify_all();
+                } else {
+                    f.C.assign(1, spg_csr{});
+                    publish(f, SPG_ERR_OUT_OF_MEMORY, "the request's output exceeds the server's memory budget");
+                }
+                continue;
+            }
+            cap = max_batch;
+            st = f.job->launch_numeric();
+            if (st == SPG_OK) st = f.job->finish();
+            publish(f, st, st == SPG_OK ? std::string() : g_last_error);
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                in_flight -= f.bytes;
+                --admitted;
+            }
+            cv_mem.notify_all();
+        }
+    }
+};
 
 }  // namespace
 }  // namespace spg
@@ -680,9 +934,9 @@ spg_status spg_csr_free(spg_csr* C) {
     std::lock_guard<std::recursive_mutex> lock(g_mutex);
     Context* c = nullptr;
     SPG_TRY(ensure_context(c));
-    SPG_TRY(dev_release(*c, C->rpt));
-    SPG_TRY(dev_release(*c, C->col));
-    SPG_TRY(dev_release(*c, C->val));
+    SPG_TRY(dev_release(*c, C->rpt, c->stream));
+    SPG_TRY(dev_release(*c, C->col, c->stream));
+    SPG_TRY(dev_release(*c, C->val, c->stream));
     *C = spg_csr{0, 0, 0, nullptr, nullptr, nullptr};
     return SPG_OK;
 }
@@ -867,6 +1121,104 @@ spg_status spg_profile_get(int32_t index, char* name, size_t name_len, int64_t* 
     if (rows) *rows = r.rows;
     if (grid) *grid = r.grid;
     if (ms) *ms = r.ms;
+    return SPG_OK;
+}
+
+spg_status spg_server_create(size_t memory_budget_bytes, int32_t max_batch, spg_server** out) {
+    if (!out || max_batch < 1 || memory_budget_bytes == 0) return fail(SPG_ERR_INVALID_ARGUMENT, "bad server arguments");
+    *out = nullptr;
+    Context* c = nullptr;
+    {
+        std::lock_guard<std::recursive_mutex> lock(g_mutex);
+        SPG_TRY(ensure_context(c));
+    }
+    auto* srv = new Server();
+    srv->c = c;
+    srv->budget = memory_budget_bytes;
+    srv->max_batch = max_batch;
+    for (auto& L : srv->lanes) {
+        const spg_status st = lane_create(L);
+        if (st != SPG_OK) {
+            for (auto& M : srv->lanes) lane_destroy(M);
+            delete srv;
+            return st;
+        }
+    }
+    for (int i = 0; i < Server::kLanes; ++i) srv->workers[i] = std::thread([srv, i] { srv->run(i); });
+    *out = reinterpret_cast<spg_server*>(srv);
+    return SPG_OK;
+}
+
+spg_status spg_server_submit(spg_server* server, const spg_csr* A, const spg_csr* B, spg_dtype dtype,
+                             int64_t* ticket) {
+    if (!server || !A || !B || !ticket) return fail(SPG_ERR_INVALID_ARGUMENT, "null argument");
+    if (dtype != SPG_F32 && dtype != SPG_F64) return fail(SPG_ERR_INVALID_ARGUMENT, "unknown dtype");
+    SPG_TRY(check_operands(*A, *B));
+    auto* srv = reinterpret_cast<Server*>(server);
+    auto r = std::make_shared<Request>();
+    r->A = *A;
+    r->B = *B;
+    r->dtype = dtype;
+    r->t_submit = now_s();
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        if (srv->stop) return fail(SPG_ERR_INVALID_ARGUMENT, "server is shutting down");
+        r->ticket = srv->next_ticket++;
+        srv->requests[r->ticket] = r;
+        srv->pending.push_back(r);
+    }
+    srv->cv_work.notify_one();
+    *ticket = r->ticket;
+    return SPG_OK;
+}
+
+spg_status spg_server_wait(spg_server* server, int64_t ticket, spg_csr* C, double* latency_ms) {
+    if (!server || !C) return fail(SPG_ERR_INVALID_ARGUMENT, "null argument");
+    auto* srv = reinterpret_cast<Server*>(server);
+    std::shared_ptr<Request> r;
+    {
+        std::unique_lock<std::mutex> lk(srv->mu);
+        auto it = srv->requests.find(ticket);
+        if (it == srv->requests.end()) return fail(SPG_ERR_INVALID_ARGUMENT, "unknown ticket");
+        r = it->second;
+        srv->cv_done.wait(lk, [&] { return r->done; });
+        srv->requests.erase(it);
+    }
+    *C = r->C;
+    if (latency_ms) *latency_ms = 1e3 * (r->t_done - r->t_submit);
+    if (r->st != SPG_OK) return fail(r->st, r->err);
+    return SPG_OK;
+}
+
+spg_status spg_server_stats(spg_server* server, int64_t* batches, int64_t* served, int64_t* max_batch_seen,
+                            int64_t* deferred, int64_t* peak_bytes) {
+    if (!server) return fail(SPG_ERR_INVALID_ARGUMENT, "null argument");
+    auto* srv = reinterpret_cast<Server*>(server);
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if (batches) *batches = srv->n_batches;
+    if (served) *served = srv->n_served;
+    if (max_batch_seen) *max_batch_seen = srv->max_seen;
+    if (deferred) *deferred = srv->n_deferred;
+    if (peak_bytes) *peak_bytes = static_cast<int64_t>(srv->peak_bytes);
+    return SPG_OK;
+}
+
+spg_status spg_server_destroy(spg_server* server) {
+    if (!server) return fail(SPG_ERR_INVALID_ARGUMENT, "null argument");
+    auto* srv = reinterpret_cast<Server*>(server);
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->stop = true;
+    }
+    srv->cv_work.notify_all();
+    for (auto& w : srv->workers)
+        if (w.joinable()) w.join();
+    for (auto& kv : srv->requests) {  // results never claimed
+        spg_csr& C = kv.second->C;
+        if (kv.second->st == SPG_OK) (void)spg_csr_free(&C);
+    }
+    for (auto& L : srv->lanes) lane_destroy(L);
+    delete srv;
     return SPG_OK;
 }
 

@@ -25,10 +25,12 @@
  *   spg_last_error      src/libutils/include/utils/errors.cuh:11-22
  *
  * Library-specific additions (no reference counterpart; needed by the
- * drop-in harness, the benchmark and the tuner):
+ * drop-in harness, the benchmark, the tuner and the two system forms that
+ * stand beside the plain call):
  *   spg_init, spg_device_info, spg_set_tuning, spg_get_tuning,
- *   spg_set_profiling, spg_profile_count, spg_profile_get,
- *   spg_multiply_batched.
+ *   spg_get_variants, spg_set_profiling, spg_profile_count, spg_profile_get,
+ *   spg_multiply_batched (many products in one pass of the kernels),
+ *   spg_server_* (a request loop with continuous batching).
  */
 #ifndef GPUSPGEMM_H
 #define GPUSPGEMM_H
@@ -122,6 +124,36 @@ spg_status spg_set_profiling(int32_t enabled);
 int32_t spg_profile_count(void);
 spg_status spg_profile_get(int32_t index, char *name, size_t name_len, int64_t *rows,
                            int32_t *grid, double *ms);
+
+/* ---- served form ----------------------------------------------------
+ * A server owns three lanes (stream sets with their own scratch), each
+ * with a worker thread.  Submitted requests queue up; a free worker takes
+ * the oldest and the next queued requests of its value type (up to
+ * max_batch requests and 4 Mi nonzeros of A; a larger request runs alone),
+ * runs their symbolic phase as one batch, and admits the batch once the
+ * exact output memory it now knows (nnz(C_i) * (4 + sizeof(value)) summed)
+ * fits the memory budget beside the batches the other lanes are running.
+ * Lanes run concurrently: arrivals are taken while earlier batches are on
+ * the GPU, and the phases of different batches overlap.
+ * A batch that cannot fit even alone is split; a single request whose
+ * output exceeds the budget fails with SPG_ERR_OUT_OF_MEMORY.
+ * Operands must stay valid until the request's spg_server_wait returns. */
+typedef struct spg_server spg_server;
+
+spg_status spg_server_create(size_t memory_budget_bytes, int32_t max_batch, spg_server **out);
+spg_status spg_server_submit(spg_server *server, const spg_csr *A, const spg_csr *B, spg_dtype dtype,
+                             int64_t *ticket);
+/* Blocks until the request is done; on success C (library-owned, free with
+ * spg_csr_free) and the latency from submit to completion in milliseconds. */
+spg_status spg_server_wait(spg_server *server, int64_t ticket, spg_csr *C, double *latency_ms);
+/* batches run, requests served, largest batch, requests sent back to the
+ * queue because their batch could not fit the budget even alone, and the
+ * largest output memory of admitted batches running at once */
+spg_status spg_server_stats(spg_server *server, int64_t *batches, int64_t *served, int64_t *max_batch_seen,
+                            int64_t *deferred, int64_t *peak_bytes);
+/* Finishes queued and running requests, then releases the server (results
+ * never waited for are freed). */
+spg_status spg_server_destroy(spg_server *server);
 
 #ifdef __cplusplus
 }

@@ -1,12 +1,16 @@
 // Driver of the CPU emulation build: C = A . B through the library's C ABI
 // (kernels emulated on the host), compared with the host Gustavson product.
 //   emu_spgemm A.csr B.csr [--f32] [--sym-table FILE] [--num-table FILE] [--repeat N]
+//              [--serve N BUDGET_BYTES MAX_BATCH]
 // Exit status 0 when the structure is identical and values agree to 1e-12
-// (FP64) / 1e-5 (FP32) after |A||B| scaling.
+// (FP64) / 1e-5 (FP32) after |A||B| scaling.  --serve submits the product N
+// times to a server (spg_server_*) and checks every result; a budget below
+// one output must fail every request with SPG_ERR_OUT_OF_MEMORY.
 #include <cstdio>
 #include <fstream>
 #include <sstream>
 #include <string>
+#include <vector>
 
 #include "gpuspgemm.hpp"
 #include "host_spgemm.hpp"
@@ -38,13 +42,69 @@ static int run(const std::string& fa, const std::string& fb, int repeat) {
     return bad;
 }
 
+template <typename T>
+static int serve(const std::string& fa, const std::string& fb, int n, long long budget, int max_batch) {
+    const auto ha = utils::HostCSR<T>::load_from_filename(fa);
+    const auto hb = utils::HostCSR<T>::load_from_filename(fb);
+    const auto da = ha.template to<utils::Location::Device>();
+    const auto db = hb.template to<utils::Location::Device>();
+    const auto ref = host::gustavson(ha, hb);
+    const auto absref = host::abs_product(ha, hb);
+    const long long need = static_cast<long long>(ref.nnz) * (4 + sizeof(T));
+    spg_server* srv = nullptr;
+    utils::handle_spg_error(spg_server_create(static_cast<size_t>(budget), max_batch, &srv), "server_create");
+    const spg_dtype dt = sizeof(T) == 8 ? SPG_F64 : SPG_F32;
+    std::vector<int64_t> tickets(n);
+    const spg_csr A = da.raw(), B = db.raw();
+    for (int i = 0; i < n; ++i) utils::handle_spg_error(spg_server_submit(srv, &A, &B, dt, &tickets[i]), "submit");
+    int bad = 0;
+    for (int i = 0; i < n; ++i) {
+        spg_csr c{};
+        double ms = 0;
+        const spg_status st = spg_server_wait(srv, tickets[i], &c, &ms);
+        if (need > budget) {
+            std::printf("request %d: status %s (output %lld bytes over budget %lld)\n", i, spg_status_str(st), need,
+                        budget);
+            if (st != SPG_ERR_OUT_OF_MEMORY) bad = 1;
+            continue;
+        }
+        if (st != SPG_OK) {
+            std::printf("request %d failed: %s\n", i, spg_last_error());
+            bad = 1;
+            continue;
+        }
+        utils::DeviceCSR<T> dc;
+        dc.nnz = c.nnz;
+        dc.m = c.m;
+        dc.n = c.n;
+        dc.rpt = c.rpt;
+        dc.col = c.col;
+        dc.val = static_cast<T*>(c.val);
+        dc.library_owned = true;
+        const auto hc = dc.template to<utils::Location::Host>();
+        const auto e = host::errors(hc, ref, absref);
+        const double tol = sizeof(T) == 8 ? 1e-12 : 1e-5;
+        std::printf("request %d: nnz %d structure %s scaled_err %.3e\n", i, hc.nnz,
+                    e.structure ? "identical" : "DIFFERENT", e.scaled);
+        if (!e.structure || e.scaled > tol) bad = 1;
+    }
+    int64_t batches = 0, served = 0, max_seen = 0, deferred = 0, peak = 0;
+    utils::handle_spg_error(spg_server_stats(srv, &batches, &served, &max_seen, &deferred, &peak), "stats");
+    std::printf("served %lld in %lld batches (largest %lld, deferred %lld, peak output bytes %lld)\n",
+                (long long)served, (long long)batches, (long long)max_seen, (long long)deferred, (long long)peak);
+    if (peak > budget) bad = 1;
+    utils::handle_spg_error(spg_server_destroy(srv), "server_destroy");
+    return bad;
+}
+
 int main(int argc, char** argv) {
     if (argc < 3) {
         std::fprintf(stderr, "usage: %s A.csr B.csr [--f32] [--sym-table F] [--num-table F] [--repeat N]\n", argv[0]);
         return 2;
     }
     bool f32 = false;
-    int repeat = 1;
+    int repeat = 1, serve_n = 0, max_batch = 1;
+    long long budget = 0;
     for (int i = 3; i < argc; ++i) {
         const std::string a = argv[i];
         if (a == "--f32") {
@@ -54,12 +114,19 @@ int main(int argc, char** argv) {
             utils::handle_spg_error(spg_set_tuning(a == "--sym-table" ? 0 : 1, t.c_str()), "set_tuning");
         } else if (a == "--repeat" && i + 1 < argc) {
             repeat = std::atoi(argv[++i]);
+        } else if (a == "--serve" && i + 3 < argc) {
+            serve_n = std::atoi(argv[++i]);
+            budget = std::atoll(argv[++i]);
+            max_batch = std::atoi(argv[++i]);
         } else {
             std::fprintf(stderr, "unknown argument %s\n", a.c_str());
             return 2;
         }
     }
     try {
+        if (serve_n > 0)
+            return f32 ? serve<float>(argv[1], argv[2], serve_n, budget, max_batch)
+                       : serve<double>(argv[1], argv[2], serve_n, budget, max_batch);
         return f32 ? run<float>(argv[1], argv[2], repeat) : run<double>(argv[1], argv[2], repeat);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "error: %s\n", e.what());

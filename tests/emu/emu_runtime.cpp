@@ -154,6 +154,10 @@ void exchange_write(std::uint64_t v) { g_slots[tl_thread.x] = v; }
 std::uint64_t exchange_read(int l) { return g_slots[(tl_thread.x / 64) * 64 + static_cast<unsigned>(l)]; }
 
 void launch(dim3 grid, dim3 block, const std::function<void()>& body) {
+    // Kernels run one at a time on the emulated device: launches from several
+    // host threads (the server's lanes) queue here.
+    static std::mutex device;
+    std::lock_guard<std::mutex> one_kernel(device);
     const int n = static_cast<int>(block.x);
     if (n < 1 || n > kMaxThreads || block.y != 1 || block.z != 1 || grid.y != 1 || grid.z != 1) std::abort();
     Pool& p = pool();

@@ -165,6 +165,7 @@ inline int atomicOr(int* p, int v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ
 // ---- runtime
 typedef int hipError_t;
 constexpr hipError_t hipSuccess = 0;
+constexpr hipError_t hipErrorNotReady = 600;
 constexpr hipError_t hipErrorInvalidValue = 1;
 constexpr hipError_t hipErrorOutOfMemory = 2;
 constexpr hipError_t hipErrorMemoryAllocation = 2;
@@ -189,6 +190,7 @@ struct hipDeviceProp_t {
 
 inline const char* hipGetErrorString(hipError_t e) { return e == hipSuccess ? "no error" : "emulated error"; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
+inline hipError_t hipSetDevice(int) { return hipSuccess; }
 inline hipError_t hipGetDevice(int* d) {
     *d = 0;
     return hipSuccess;
@@ -212,6 +214,12 @@ inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) {
 }
 inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+// work runs synchronously in the emulation: every stream and event is complete
+inline hipError_t hipStreamQuery(hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventQuery(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
 inline hipError_t hipDeviceGetDefaultMemPool(hipMemPool_t*, int) { return hipErrorNotSupported; }
 inline hipError_t hipMemPoolSetAttribute(hipMemPool_t, hipMemPoolAttr, void*) { return hipSuccess; }

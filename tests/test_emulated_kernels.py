@@ -22,6 +22,7 @@ CSRC = os.path.join(ROOT, "gpuspgemm_amd", "csrc")
 INT_MAX = 2**31 - 1
 
 import gpuspgemm_amd as g
+import oracle
 
 # Compiled variants, read from the library (no GPU needed): symbolic
 # (lanes per row, A entries per chunk, table slots) and fused (lanes, chunk,
@@ -133,3 +134,21 @@ def test_global_paths(emu, tmp_path):
     run_emu(emu, tmp_path, A, B)
     run_emu(emu, tmp_path, A, B,
             tables={0: f"{SYM_MAX} {INT_MAX} {INT_MAX}\n", 1: f"{NUM_GLOBAL} {INT_MAX} {INT_MAX}\n"})
+
+
+def test_server_batches_and_admission(emu, tmp_path):
+    """Served form on the CPU: the same product submitted many times to a
+    server with a small maximum batch and a budget of about two outputs,
+    so batches are split, admitted one after another and run on both
+    lanes; every result must equal the host product.  A budget below one
+    output must fail every request with out-of-memory."""
+    A = gen.powerlaw(m=3000, seed=8)
+    C = oracle.spgemm(A, A)
+    need = C.nnz * 12
+    out = run_emu(emu, tmp_path, A, A, "--serve", "7", str(2 * need + 64), "3")
+    assert out.count("structure identical") == 7
+    assert "served 7" in out
+    res = subprocess.run([emu, str(tmp_path / "a.csr"), str(tmp_path / "b.csr"), "--serve", "2", str(need // 2), "2"],
+                         stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=600)
+    assert res.returncode == 0, res.stdout[-3000:]
+    assert res.stdout.count("out of device memory") == 2
