@@ -18,6 +18,7 @@
 #include "../../include/gpuspgemm.h"
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -83,6 +84,7 @@ struct Lane {
     hipStream_t aux[kStreams - 1] = {};
     hipEvent_t ev_ready = nullptr;
     hipEvent_t ev_done[kStreams - 1] = {};
+    hipEvent_t ev_host = nullptr;  // the host spins on it at the two read-backs
     char* scratch = nullptr;
     size_t scratch_cap = 0;
     DevMeta* h_meta = nullptr;
@@ -183,6 +185,7 @@ spg_status lane_create(Lane& L) {
         SPG_HIP(hipEventCreateWithFlags(&L.ev_done[i], hipEventDisableTiming));
     }
     SPG_HIP(hipEventCreateWithFlags(&L.ev_ready, hipEventDisableTiming));
+    SPG_HIP(hipEventCreateWithFlags(&L.ev_host, hipEventDisableTiming));
     SPG_HIP(hipHostMalloc(reinterpret_cast<void**>(&L.h_meta), sizeof(DevMeta), hipHostMallocDefault));
     return SPG_OK;
 }
@@ -195,11 +198,23 @@ void lane_destroy(Lane& L) {
         if (L.ev_done[i]) (void)hipEventDestroy(L.ev_done[i]);
     }
     if (L.ev_ready) (void)hipEventDestroy(L.ev_ready);
+    if (L.ev_host) (void)hipEventDestroy(L.ev_host);
     if (L.scratch) (void)hipFree(L.scratch);
     if (L.h_meta) (void)hipHostFree(L.h_meta);
     if (L.h_stage) (void)hipHostFree(L.h_stage);
     if (L.s) (void)hipStreamDestroy(L.s);
     L = Lane{};
+}
+
+// Wait for everything queued on the lane's main stream by polling an event:
+// a blocking synchronisation may put the host thread to sleep, and waking
+// it costs tens of microseconds at each of the multiply's two read-backs.
+hipError_t lane_wait(Lane& L) {
+    hipError_t e = hipEventRecord(L.ev_host, L.s);
+    if (e != hipSuccess) return e;
+    while ((e = hipEventQuery(L.ev_host)) == hipErrorNotReady) {
+    }
+    return e;
 }
 
 spg_status ensure_context(Context*& out) {
@@ -476,7 +491,7 @@ struct Job : JobBase {
         if (batch)
             SPG_HIP(hipMemcpyAsync(L.h_stage + stage_inst, inst_nnz, sizeof(long long) * count, hipMemcpyDeviceToHost,
                                    s));
-        SPG_HIP(hipStreamSynchronize(s));
+        SPG_HIP(lane_wait(L));
         return SPG_OK;
     }
 
@@ -668,13 +683,25 @@ struct Job : JobBase {
                 item_off += cnt;
             }
         }
+        // Stream plan: launches in order of decreasing row capacity times
+        // lanes per row, dealt round robin over the lane's streams.  The
+        // heavy-row launches (few workgroups, each a long chain) are queued
+        // first, so their workgroups reach the CUs before the launches of
+        // many small rows fill them, and then run beside those.  (Measured
+        // against a longest-first plan and a plan that keeps the
+        // GPU-filling launches on one stream: this one was fastest.)
         std::stable_sort(launches.begin(), launches.end(), [](const Launch& x, const Launch& y) {
             return kNumVariants[x.variant].p * kNumVariants[x.variant].wg / kNumVariants[x.variant].r >
                    kNumVariants[y.variant].p * kNumVariants[y.variant].wg / kNumVariants[y.variant].r;
         });
-        const int n_launch = static_cast<int>(launches.size()) + (glob_bin >= 0 ? 1 : 0);
-        const int n_streams = std::min(n_launch, static_cast<int>(Lane::kStreams));
-        auto stream_of = [&](int k) { return k % Lane::kStreams == 0 ? s : L.aux[k % Lane::kStreams - 1]; };
+        std::vector<int> stream_idx(launches.size());
+        int n_streams = 1;
+        {
+            int k = glob_bin >= 0 ? 1 : 0;  // the global bin, when present, is queued first on the main stream
+            for (size_t i = 0; i < launches.size(); ++i, ++k) stream_idx[i] = k % Lane::kStreams;
+            n_streams = std::min(std::max(k, 1), static_cast<int>(Lane::kStreams));
+        }
+        auto stream_of = [&](int q) { return q == 0 ? s : L.aux[q - 1]; };
         if (glob_bin >= 0) {
             const spg_status st = dev_alloc(c, &num_pool, sizeof(int32_t) * hm.gnum_pool, s);
             if (st != SPG_OK) return failed(st);
@@ -684,21 +711,20 @@ struct Job : JobBase {
             e = hipEventRecord(L.ev_ready, s);
             for (int i = 1; i < n_streams && e == hipSuccess; ++i) e = hipStreamWaitEvent(L.aux[i - 1], L.ev_ready, 0);
         }
-        int k = 0;
         if (e == hipSuccess && glob_bin >= 0) {
-            const hipStream_t on = stream_of(k++);
+            const hipStream_t on = stream_of(0);
             prof.begin("symnum_global", hm.num_count[glob_bin], hm.num_count[glob_bin], on);
             launch_symnum_global<V>(ps, items2, gpool_off, glob_off, hm.num_count[glob_bin],
                                     static_cast<int*>(num_pool), meta, on);
             prof.end();
         }
-        for (const Launch& Lh : launches) {
+        for (size_t li = 0; li < launches.size(); ++li) {
             if (e != hipSuccess) break;
+            const Launch& Lh = launches[li];
             const NumVariant& nv = kNumVariants[Lh.variant];
             const int units = (Lh.count + nv.r - 1) / nv.r;
-            int grid = std::min(units, c.cus * c.occ_num[di][Lh.variant]);
-            grid = (grid + kXcds - 1) / kXcds * kXcds;
-            const hipStream_t on = stream_of(k++);
+            const int grid = (units + kXcds - 1) / kXcds * kXcds;  // one unit per workgroup
+            const hipStream_t on = stream_of(stream_idx[li]);
             prof.begin("symnum_v" + std::to_string(Lh.variant), Lh.count, grid, on);
             e = launch_symnum_bin<V>(Lh.variant, ps, items2, Lh.off, Lh.count, meta, grid, on);
             prof.end();
@@ -719,7 +745,7 @@ struct Job : JobBase {
     }
 
     spg_status finish() override {
-        const hipError_t e = hipStreamSynchronize(L.s);
+        const hipError_t e = lane_wait(L);
         if (e != hipSuccess) return failed(hip_fail(e, "fused pass", __FILE__, __LINE__));
         if (!empty && (L.h_meta->flags.v & kFlagMismatch))
             return failed(fail(SPG_ERR_INTERNAL, "fused pass found a different column count than the count pass "

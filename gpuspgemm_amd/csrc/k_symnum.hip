@@ -51,8 +51,8 @@ __device__ __forceinline__ int compact_table(const int* tab, int* cols, int lr, 
 // Sorted column list of a row: the occupied slots of its table in
 // ascending order in cols[0..n), padded with INT_MAX up to P; returns n.
 // Groups inside one wave sort in registers (bitonic_sort_regs) and also
-// leave lane lr's share cols[lr*E .. lr*E + E) in x; larger groups run the
-// bitonic network in LDS.
+// leave lane lr's share cols[lr*E .. lr*E + E) in x; workgroup-wide rows
+// sort wave blocks in registers and merge them in LDS (bitonic_sort_block).
 template <int L, int T, int P>
 __device__ __forceinline__ int sort_row(const int* tab, int* cols, int lr, int* s_tmp, int (&x)[L <= kWave ? P / L : 1]) {
     const int found = compact_table<L, T, P>(tab, cols, lr, s_tmp);
@@ -69,7 +69,7 @@ __device__ __forceinline__ int sort_row(const int* tab, int* cols, int lr, int* 
     } else {
         for (int j = n + lr; j < P; j += L) cols[j] = INT_MAX;
         gsync<L>();
-        bitonic_sort<P, L>(cols, lr);
+        bitonic_sort_block<P, L>(cols, lr);
     }
     return found;
 }
@@ -175,8 +175,10 @@ __device__ __forceinline__ void symnum_rows(const ProbSet<V>& ps, const int2* __
     gsync<L>();  // LDS is reused by the next unit
 }
 
-// One launch per fused variant: a persistent grid walks the units (R rows
-// each) of one bin; consecutive units stay on one XCD.
+// One launch per fused variant over the units (R rows each) of one bin; the
+// host launches a workgroup per unit (residency is the hardware's), and the
+// unit of a workgroup is chosen so that consecutive units stay on one XCD.
+// A smaller grid walks the units persistently.
 template <typename V, int VV>
 __global__ __launch_bounds__(kNumVariants[VV].wg) void k_symnum_bin(ProbSet<V> ps, const int2* __restrict__ items,
                                                                     int item_off, int count, DevMeta* meta) {
@@ -278,7 +280,7 @@ __global__ __launch_bounds__(1024) void k_symnum_global(ProbSet<V> ps, const int
             chunk[j] = e < n ? __hip_atomic_load(ccol + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : INT_MAX;
         }
         __syncthreads();
-        bitonic_sort<CH, WG>(chunk, threadIdx.x);
+        bitonic_sort_block<CH, WG>(chunk, threadIdx.x);
         for (int j = threadIdx.x; j < CH; j += WG) tab[static_cast<long long>(c) * CH + j] = chunk[j];
         __syncthreads();
     }

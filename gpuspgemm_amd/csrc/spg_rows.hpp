@@ -258,12 +258,90 @@ __device__ __forceinline__ void lds_fill(int* p, int v, int lr) {
     }
 }
 
-// Ascending bitonic sort of P ints (P a power of two) in LDS by the L lanes
-// of a row group.
+// Bitonic sorting network on P ints held in registers by the L lanes of a
+// group (L <= 64): lane lr holds elements base + lr*E .. base + lr*E + E-1
+// in x[0..E), E = P / L.  `base` is the index of the group's first element
+// in a larger network: the direction of every compare-exchange follows the
+// global index, so groups can run the first levels of a network that
+// several groups finish together.  Stages whose partner lies in the same
+// lane are register compare-exchanges; the others exchange with lane
+// lr ^ (j / E) through cross-lane shuffles, E of them in flight at once.
+
+// One stage (level k, distance J) of the network.
+template <int E, int L, int J>
+__device__ __forceinline__ void bitonic_step_regs(int (&x)[E], int lr, int base, int k) {
+    if constexpr (J >= E) {
+        constexpr int LJ = J / E;
+        const bool lower = (lr & LJ) == 0;
+        int other[E];
+#pragma unroll
+        for (int s = 0; s < E; ++s) other[s] = __shfl_xor(x[s], LJ, L);
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const bool asc = ((base + lr * E + s) & k) == 0;
+            x[s] = (lower == asc) ? min(x[s], other[s]) : max(x[s], other[s]);
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            if (s & J) continue;
+            const bool asc = ((base + lr * E + s) & k) == 0;
+            const int lo = min(x[s], x[s + J]);
+            const int hi = max(x[s], x[s + J]);
+            x[s] = asc ? lo : hi;
+            x[s + J] = asc ? hi : lo;
+        }
+    }
+}
+
+// Stages J, J/2, ..., 1 of level k.
+template <int E, int L, int J>
+__device__ __forceinline__ void bitonic_merge_regs(int (&x)[E], int lr, int base, int k) {
+    if constexpr (J >= 1) {
+        bitonic_step_regs<E, L, J>(x, lr, base, k);
+        bitonic_merge_regs<E, L, J / 2>(x, lr, base, k);
+    }
+}
+
+// Levels 2, 4, ..., K: a full sort of the group's P elements when K = P
+// (ascending when base is a multiple of 2P).
+template <int E, int L, int K>
+__device__ __forceinline__ void bitonic_levels_regs(int (&x)[E], int lr, int base) {
+    if constexpr (K >= 2) {
+        bitonic_levels_regs<E, L, K / 2>(x, lr, base);
+        bitonic_merge_regs<E, L, K / 2>(x, lr, base, K);
+    }
+}
+
 template <int P, int L>
-__device__ __forceinline__ void bitonic_sort(int* cols, int lr) {
-    for (int k = 2; k <= P; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
+__device__ __forceinline__ void bitonic_sort_regs(int (&x)[P / L], int lr, int base = 0) {
+    constexpr int E = P / L;
+    static_assert(E >= 1 && E * L == P && L <= kWave, "bad register sort shape");
+    bitonic_levels_regs<E, L, P>(x, lr, base);
+}
+
+// Ascending bitonic sort of P ints in LDS by a workgroup of L > 64 lanes.
+// Every wave sorts its block of B = P / waves elements in registers; each
+// further level runs its cross-block stages in LDS (a barrier each) and
+// its in-block stages in registers again.  For P = 8192 on 1024 lanes this
+// is 10 LDS stages and 5 register passes instead of 91 LDS stages.
+template <int P, int L>
+__device__ __forceinline__ void bitonic_sort_block(int* cols, int lr) {
+    constexpr int W = L / kWave;
+    constexpr int B = P / W;
+    constexpr int E = B / kWave;
+    static_assert(L > kWave && L % kWave == 0 && E >= 1 && E * kWave * W == P, "bad block sort shape");
+    const int lane = lr % kWave;
+    const int base = (lr / kWave) * B;
+    int x[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) x[s] = cols[base + lane * E + s];
+    bitonic_sort_regs<B, kWave>(x, lane, base);
+#pragma unroll
+    for (int s = 0; s < E; ++s) cols[base + lane * E + s] = x[s];
+    __syncthreads();
+    for (int k = 2 * B; k <= P; k <<= 1) {
+        for (int j = k >> 1; j >= B; j >>= 1) {
             for (int t = lr; t < P / 2; t += L) {
                 const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
                 const int a = cols[i];
@@ -274,47 +352,14 @@ __device__ __forceinline__ void bitonic_sort(int* cols, int lr) {
                     cols[i + j] = a;
                 }
             }
-            gsync<L>();
+            __syncthreads();
         }
-    }
-}
-
-// Ascending bitonic sort of P ints held in registers by the L lanes of a
-// row group (L <= 64, P a power of two, P >= L): lane lr holds elements
-// lr * E .. lr * E + E - 1 in x[0..E).  Stages whose partner lies in the
-// same lane are register compare-exchanges; the others exchange with lane
-// lr ^ (j / E) through cross-lane shuffles, E of them in flight at once.
-template <int P, int L>
-__device__ __forceinline__ void bitonic_sort_regs(int (&x)[P / L], int lr) {
-    constexpr int E = P / L;
-    static_assert(E >= 1 && E * L == P && L <= kWave, "bad register sort shape");
 #pragma unroll
-    for (int k = 2; k <= P; k <<= 1) {
+        for (int s = 0; s < E; ++s) x[s] = cols[base + lane * E + s];
+        bitonic_merge_regs<E, kWave, B / 2>(x, lane, base, k);
 #pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j >= E) {
-                const int lj = j / E;
-                const bool lower = (lr & lj) == 0;
-                int other[E];
-#pragma unroll
-                for (int s = 0; s < E; ++s) other[s] = __shfl_xor(x[s], lj, L);
-#pragma unroll
-                for (int s = 0; s < E; ++s) {
-                    const bool asc = ((lr * E + s) & k) == 0;
-                    x[s] = (lower == asc) ? min(x[s], other[s]) : max(x[s], other[s]);
-                }
-            } else {
-#pragma unroll
-                for (int s = 0; s < E; ++s) {
-                    if (s & j) continue;
-                    const bool asc = ((lr * E + s) & k) == 0;
-                    const int lo = min(x[s], x[s + j]);
-                    const int hi = max(x[s], x[s + j]);
-                    x[s] = asc ? lo : hi;
-                    x[s + j] = asc ? hi : lo;
-                }
-            }
-        }
+        for (int s = 0; s < E; ++s) cols[base + lane * E + s] = x[s];
+        __syncthreads();
     }
 }
 
