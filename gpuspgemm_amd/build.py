@@ -60,15 +60,30 @@ def build_lib() -> str:
 
 
 def build_apps() -> list[str]:
+    """bin/proposal(_f32): the library's app (plain C++, no HIP headers);
+    bin/rocsparse(_f32): the vendor-library comparator (host code against
+    the HIP runtime and rocSPARSE)."""
     os.makedirs(BIN, exist_ok=True)
     outs = []
+    cmds = []
     for name, defs in (("proposal", ["-DUSE_DOUBLE_PRECISION"]), ("proposal_f32", [])):
         out = os.path.join(BIN, name)
-        _run(["g++", "-std=c++20", "-O2", "-Wall", "-Wextra", *defs,
-              "-I", os.path.join(ROOT, "include"),
-              os.path.join(PKG, "apps", "proposal.cpp"), "-o", out,
-              "-L", PKG, "-lgpuspgemm", "-Wl,-rpath,$ORIGIN/..", "-pthread"])
+        cmds.append(["g++", "-std=c++20", "-O2", "-Wall", "-Wextra", *defs,
+                     "-I", os.path.join(ROOT, "include"),
+                     os.path.join(PKG, "apps", "proposal.cpp"), "-o", out,
+                     "-L", PKG, "-lgpuspgemm", "-Wl,-rpath,$ORIGIN/..", "-pthread"])
         outs.append(out)
+    rocm = os.path.dirname(os.path.dirname(os.path.realpath(HIPCC)))
+    for name, defs in (("rocsparse", ["-DUSE_DOUBLE_PRECISION"]), ("rocsparse_f32", [])):
+        out = os.path.join(BIN, name)
+        cmds.append([HIPCC, f"--offload-arch={ARCH}", "-std=c++20", "-O2", *defs,
+                     "-I", os.path.join(ROOT, "include"),
+                     os.path.join(PKG, "apps", "rocsparse.cpp"), "-o", out,
+                     "-L", PKG, "-lgpuspgemm", "-L", os.path.join(rocm, "lib"), "-lrocsparse",
+                     "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath," + os.path.join(rocm, "lib"), "-pthread"])
+        outs.append(out)
+    with cf.ThreadPoolExecutor(max_workers=4) as ex:
+        list(ex.map(_run, cmds))
     return outs
 
 

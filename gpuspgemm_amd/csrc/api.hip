@@ -104,7 +104,7 @@ struct Context {
     bool async_alloc = true;
     BinTable sym{};
     BinTable num{};
-    int occ_sym_multi = 1, occ_sym_max = 1;
+    int occ_sym_multi[2] = {1, 1}, occ_sym_max = 1;
     int occ_num[2][kNumLdsVariants] = {};
     Lane lane;
     bool profiling = false;
@@ -250,7 +250,8 @@ spg_status ensure_context(Context*& out) {
     default_tables(c.sym, c.num);
     SPG_TRY(check_sym_table(c.sym));  // a stale tuned_tables.inc fails here, loudly
     SPG_TRY(check_num_table(c.num));
-    c.occ_sym_multi = occupancy_sym_multi<double>();
+    c.occ_sym_multi[0] = occupancy_sym_multi<double>(0);
+    c.occ_sym_multi[1] = occupancy_sym_multi<double>(1);
     c.occ_sym_max = occupancy_sym_max<double>();
     for (int v = 0; v < kNumLdsVariants; ++v) {
         c.occ_num[0][v] = occupancy_symnum_bin<float>(v);
@@ -467,11 +468,14 @@ struct Job : JobBase {
             prof.end();
         }
         const int n_multi = sym.n - 1;
-        if (n_multi > 0) {
-            int grid = c.cus * c.occ_sym_multi;
+        bool part_used[2] = {false, false};
+        for (int b = 0; b < n_multi; ++b) part_used[sym.variant[b] < kSymSmallVariants ? 0 : 1] = true;
+        for (int part = 0; part < 2; ++part) {
+            if (!part_used[part]) continue;
+            int grid = c.cus * c.occ_sym_multi[part];
             grid = (grid + kXcds - 1) / kXcds * kXcds;
-            prof.begin("sym_multi", total_rows, grid, s);
-            launch_sym_multi<V>(ps, items1, sym, n_multi, meta, grid, s);
+            prof.begin(part == 0 ? "sym_multi_small" : "sym_multi", total_rows, grid, s);
+            launch_sym_multi<V>(part, ps, items1, sym, n_multi, meta, grid, s);
             prof.end();
         }
         SPG_HIP(hipEventRecord(L.ev_done[0], L.aux[0]));

@@ -52,18 +52,34 @@ __device__ __forceinline__ void sym_rows(const ProbSet<V>& ps, const int2* __res
 
 #define SPG_SYM_CASE(v)                                                                                   \
     case v:                                                                                               \
-        sym_rows<V, kMultiThreads, kSymVariants[v].r, kSymVariants[v].t, kSymVariants[v].ec,              \
-                 kSymVariants[v].pmax>(ps, items, first, end, smem, s_tmp);                               \
+        if constexpr (v >= LO && v < HI)                                                                  \
+            sym_rows<V, kMultiThreads, kSymVariants[v].r, kSymVariants[v].t, kSymVariants[v].ec,          \
+                     kSymVariants[v].pmax>(ps, items, first, end, smem, s_tmp);                           \
         break;
 
-// One launch serves bins [0, n_multi) of the symbolic table.  Bin sizes are
-// read on the device, so the launch needs no host round trip; the grid is
-// persistent and walks the units of all bins.
-template <typename V>
+// LDS of a multi-bin launch over variants [LO, HI).
+template <int LO, int HI>
+constexpr int sym_multi_lds() {
+    int m = 0;
+    for (int i = LO; i < HI; ++i) {
+        const int b = 4 * kSymVariants[i].r * sym_row_ints(kSymVariants[i]);
+        m = m > b ? m : b;
+    }
+    return m;
+}
+
+// One launch serves the bins among [0, n_multi) of the symbolic table whose
+// variant lies in [LO, HI).  Bin sizes are read on the device, so the
+// launch needs no host round trip; the grid is persistent and walks the
+// units of its bins.  The small-table variants get a launch of their own:
+// a kernel takes the register budget of its largest variant, and theirs is
+// far smaller, so twice as many of their waves fit on a CU.
+template <typename V, int LO, int HI>
 __global__ __launch_bounds__(kMultiThreads) void k_sym_multi(ProbSet<V> ps, const int2* __restrict__ items,
                                                              BinTable sym, int n_multi,
                                                              const DevMeta* __restrict__ meta) {
-    __shared__ __align__(16) int smem[kSymMultiLds / 4];
+    static_assert(sym_multi_lds<LO, HI>() <= kSymMultiLds, "symbolic multi-bin LDS budget");
+    __shared__ __align__(16) int smem[sym_multi_lds<LO, HI>() / 4];
     __shared__ int s_tmp[kMultiThreads / kWave];
     __shared__ int s_unit[kMaxBins + 1];
     __shared__ int s_off[kMaxBins];
@@ -72,11 +88,12 @@ __global__ __launch_bounds__(kMultiThreads) void k_sym_multi(ProbSet<V> ps, cons
         int off = 0, units = 0;
         for (int b = 0; b < n_multi; ++b) {
             const int c = meta->sym_count[b].v;
+            const bool mine = sym.variant[b] >= LO && sym.variant[b] < HI;
             s_off[b] = off;
             s_cnt[b] = c;
             s_unit[b] = units;
             off += c;
-            units += (c + sym.rows[b] - 1) / sym.rows[b];
+            if (mine) units += (c + sym.rows[b] - 1) / sym.rows[b];
         }
         s_unit[n_multi] = units;
     }
@@ -215,9 +232,14 @@ __global__ __launch_bounds__(1024) void k_sym_global(ProbSet<V> ps, const int2* 
 // ----------------------------------------------------------------- launchers
 
 template <typename V>
-void launch_sym_multi(const ProbSet<V>& ps, const int2* items, const BinTable& sym, int n_multi,
+void launch_sym_multi(int part, const ProbSet<V>& ps, const int2* items, const BinTable& sym, int n_multi,
                       const DevMeta* meta, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_sym_multi<V>, dim3(grid), dim3(kMultiThreads), 0, s, ps, items, sym, n_multi, meta);
+    if (part == 0)
+        hipLaunchKernelGGL((k_sym_multi<V, 0, kSymSmallVariants>), dim3(grid), dim3(kMultiThreads), 0, s, ps, items,
+                           sym, n_multi, meta);
+    else
+        hipLaunchKernelGGL((k_sym_multi<V, kSymSmallVariants, kSymMultiVariants>), dim3(grid), dim3(kMultiThreads), 0,
+                           s, ps, items, sym, n_multi, meta);
 }
 
 template <typename V>
@@ -234,9 +256,14 @@ void launch_sym_global(const ProbSet<V>& ps, const int2* fail_items, const long 
 }
 
 template <typename V>
-int occupancy_sym_multi() {
+int occupancy_sym_multi(int part) {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sym_multi<V>, kMultiThreads, 0) != hipSuccess) n = 1;
+    const hipError_t e =
+        part == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sym_multi<V, 0, kSymSmallVariants>,
+                                                                 kMultiThreads, 0)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &n, k_sym_multi<V, kSymSmallVariants, kSymMultiVariants>, kMultiThreads, 0);
+    if (e != hipSuccess) n = 1;
     return n > 0 ? n : 1;
 }
 
@@ -248,13 +275,13 @@ int occupancy_sym_max() {
 }
 
 #define SPG_INSTANTIATE_SYM(V)                                                                           \
-    template void launch_sym_multi<V>(const ProbSet<V>&, const int2*, const BinTable&, int, const DevMeta*, \
-                                      int, hipStream_t);                                                  \
+    template void launch_sym_multi<V>(int, const ProbSet<V>&, const int2*, const BinTable&, int,           \
+                                      const DevMeta*, int, hipStream_t);                                  \
     template void launch_sym_max<V>(const ProbSet<V>&, const int2*, const BinTable&, int, DevMeta*, int2*,  \
                                     long long*, int, hipStream_t);                                        \
     template void launch_sym_global<V>(const ProbSet<V>&, const int2*, const long long*, int*, int,        \
                                        hipStream_t);                                                      \
-    template int occupancy_sym_multi<V>();                                                                \
+    template int occupancy_sym_multi<V>(int);                                                             \
     template int occupancy_sym_max<V>();
 
 SPG_INSTANTIATE_SYM(float)

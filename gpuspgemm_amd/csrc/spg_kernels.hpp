@@ -28,6 +28,7 @@ constexpr SymVariant kSymVariants[] = {
     {256, 1, 8192, 256, 8}, {1024, 1, 32768, 1024, 4},
 };
 constexpr int kSymMultiVariants = 9;
+constexpr int kSymSmallVariants = 4;  // variants 0..3 run in a multi-bin launch of their own
 constexpr int kSymMaxVariant = 9;
 constexpr int kSymMaxThreshold = 26214;  // distinct keys admitted in the 32768-slot table (80 %)
 constexpr int kSymMultiLds = 36 * 1024;  // bytes: max over variants 0..8 of r * sym_row_ints * 4
@@ -114,8 +115,9 @@ void launch_scan_apply(const ProbSet<V>& ps, const BinTable& num, DevMeta* meta,
                        int2* items, long long* gpool_off, long long* inst_nnz, int after_fail, hipStream_t s);
 
 // symbolic count: multi-bin launch over bins [0, n_multi) of `sym`
+// part 0: variants [0, kSymSmallVariants); part 1: the other multi-bin variants
 template <typename V>
-void launch_sym_multi(const ProbSet<V>& ps, const int2* items, const BinTable& sym, int n_multi,
+void launch_sym_multi(int part, const ProbSet<V>& ps, const int2* items, const BinTable& sym, int n_multi,
                       const DevMeta* meta, int grid, hipStream_t s);
 template <typename V>
 void launch_sym_max(const ProbSet<V>& ps, const int2* items, const BinTable& sym, int bin, DevMeta* meta,
@@ -134,7 +136,7 @@ void launch_symnum_global(const ProbSet<V>& ps, const int2* items, const long lo
 
 // occupancy (workgroups per CU) of the persistent launches
 template <typename V>
-int occupancy_sym_multi();
+int occupancy_sym_multi(int part);
 template <typename V>
 int occupancy_sym_max();
 template <typename V>
