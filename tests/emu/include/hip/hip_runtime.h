@@ -125,6 +125,7 @@ inline int __ffsll(long long x) { return __builtin_ffsll(x); }
 #define __HIP_MEMORY_SCOPE_WAVEFRONT 1
 #define __HIP_MEMORY_SCOPE_WORKGROUP 2
 #define __HIP_MEMORY_SCOPE_AGENT 3
+#define __HIP_MEMORY_SCOPE_SYSTEM 4
 template <typename T>
 inline T __hip_atomic_load(const T* p, int, int) {
     return __atomic_load_n(p, __ATOMIC_SEQ_CST);
@@ -166,6 +167,7 @@ inline int atomicOr(int* p, int v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ
 typedef int hipError_t;
 constexpr hipError_t hipSuccess = 0;
 constexpr hipError_t hipErrorNotReady = 600;
+constexpr hipError_t hipErrorUnknown = 999;
 constexpr hipError_t hipErrorInvalidValue = 1;
 constexpr hipError_t hipErrorOutOfMemory = 2;
 constexpr hipError_t hipErrorMemoryAllocation = 2;
