@@ -86,9 +86,15 @@ __global__ __launch_bounds__(kRowThreads) void k_nip(ProbSet<V> ps, BinTable sym
             const int la0 = __shfl(a0[j], src);
             const int la1 = __shfl(a1[j], src);
             long long part = 0;
-            for (int p = la0 + lane_id(); p < la1; p += kWave) {
-                const int kk = P.a_col[p];
-                part += P.b_rpt[kk + 1] - P.b_rpt[kk];
+            // 8 entries per lane in flight: a row of 2000 entries costs four
+            // rounds of two dependent loads instead of 32
+            for (int base = la0 + lane_id(); base < la1; base += 8 * kWave) {
+                int kk[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) kk[q] = base + q * kWave < la1 ? P.a_col[base + q * kWave] : -1;
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (kk[q] >= 0) part += P.b_rpt[kk[q] + 1] - P.b_rpt[kk[q]];
             }
             part = wave_sum64(part);
             if (lane_id() == src) s[j] = part;
@@ -339,8 +345,16 @@ __global__ __launch_bounds__(kRowThreads) void k_total_nip(const int32_t* __rest
     long long s = 0;
     for (int j = 0; j < kTileRows / kRowThreads; ++j) {
         const int row = blockIdx.x * kTileRows + j * kRowThreads + threadIdx.x;
-        if (row < m)
-            for (int p = a_rpt[row]; p < a_rpt[row + 1]; ++p) s += b_rpt[a_col[p] + 1] - b_rpt[a_col[p]];
+        if (row >= m) continue;
+        const int a1 = a_rpt[row + 1];
+        for (int base = a_rpt[row]; base < a1; base += 8) {  // 8 entries' loads in flight
+            int kk[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) kk[q] = base + q < a1 ? a_col[base + q] : -1;
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (kk[q] >= 0) s += b_rpt[kk[q] + 1] - b_rpt[kk[q]];
+        }
     }
     s = wave_sum64(s);
     if (lane_id() == 0 && s) atomicAdd(&s_total, static_cast<unsigned long long>(s));
