@@ -11,9 +11,11 @@ by measurement on the GPU it runs on:
     compiled variant whose table / capacity fits the slot's caps is a
     candidate for it;
   * slots are tuned one at a time, greedily, in two passes: each candidate
-    replaces the slot's variant and the whole multiply is timed (median of
-    `reps` calls after warmups) on every tuning workload; the score is the
-    sum over workloads of time / time with the starting tables;
+    replaces the slot's variant, or moves one of the slot's caps (halved or
+    doubled, where the variant still admits it), and the whole multiply is
+    timed (median of `reps` calls after warmups) on every tuning workload;
+    the score is the sum over workloads of time / time with the starting
+    tables;
   * the best tables are printed and, with --write, written as a C++ include
     (csrc/tuned_tables.inc) that the library compiles in as its defaults.
 
@@ -29,7 +31,7 @@ import time
 import numpy as np
 
 from . import generators as gen
-from . import device_info, get_tuning, load_library, proposal, proposal_batched, set_tuning, variant_list
+from . import SpgError, device_info, get_tuning, load_library, proposal, proposal_batched, set_tuning, variant_list
 
 INT_MAX = 2**31 - 1
 
@@ -96,6 +98,26 @@ class Workload:
             b.free()
 
 
+def cap_moves(phase: int, slot: tuple[int, int, int], variants: list[dict]) -> list[tuple[int, int, int]]:
+    """The slot with one cap halved or doubled, where the variant allows it."""
+    v, nip, nnz = slot
+    out = []
+    info = next(x for x in variants if x["id"] == v)
+    if phase == 0:
+        for c in (nip // 2, nip * 2):
+            if 1 <= c < info["table"] and c != nip:
+                out.append((v, c, nnz))
+    else:
+        if nip != INT_MAX:
+            for c in (nip // 2, nip * 2):
+                if c >= 1 and c != nip:
+                    out.append((v, c, nnz))
+        for c in (nnz // 2, nnz * 2):
+            if 1 <= c <= info["capacity"] and c != nnz:
+                out.append((v, nip, c))
+    return out
+
+
 def default_workloads(quick: bool = False) -> list[Workload]:
     P = gen.powerlaw(m=250_000 if quick else 1_000_000)
     S = gen.stencil27(g=64 if quick else 96)
@@ -117,8 +139,11 @@ def tune(workloads: list[Workload], reps: int = 5, passes: int = 2, log=print) -
     log(f"start: " + ", ".join(f"{k} {1e3 * v:.3f} ms" for k, v in base.items()))
 
     def score(t0, t1) -> tuple[float, dict]:
-        set_tuning(0, table_text(t0))
-        set_tuning(1, table_text(t1))
+        try:
+            set_tuning(0, table_text(t0))
+            set_tuning(1, table_text(t1))
+        except SpgError:  # a move that breaks the table rules
+            return float("inf"), {}
         ts = {w.name: w.time(reps) for w in workloads}
         return sum(ts[k] / base[k] for k in ts), ts
 
@@ -128,19 +153,20 @@ def tune(workloads: list[Workload], reps: int = 5, passes: int = 2, log=print) -
         for phase in (1, 0):
             for i in range(len(tables[phase]) - 1):  # the last bin (max / global variant) is fixed
                 slot = tables[phase][i]
-                cands = (sym_candidates if phase == 0 else num_candidates)(slot, sv if phase == 0 else nv)
-                for v in cands:
-                    if v == slot[0]:
-                        continue
+                vs = sv if phase == 0 else nv
+                cands = [(v, slot[1], slot[2]) for v in (sym_candidates if phase == 0 else num_candidates)(slot, vs)
+                         if v != slot[0]]
+                cands += cap_moves(phase, slot, vs)
+                for cand in cands:
                     trial = {p: list(t) for p, t in tables.items()}
-                    trial[phase][i] = (v, slot[1], slot[2])
+                    trial[phase][i] = cand
                     s, ts = score(trial[0], trial[1])
-                    trials.append({"phase": phase, "slot": i, "variant": v, "score": s, "ms": {
+                    trials.append({"phase": phase, "slot": i, "bin": list(cand), "score": s, "ms": {
                         k: round(1e3 * x, 4) for k, x in ts.items()}})
                     if s < best * 0.99:  # keep a change only if it wins by more than noise
-                        log(f"phase {phase} slot {i} ({slot[1]}, {slot[2]}): variant {slot[0]} -> {v}, "
-                            f"score {best:.4f} -> {s:.4f}")
+                        log(f"phase {phase} slot {i}: {slot} -> {cand}, score {best:.4f} -> {s:.4f}")
                         best, best_ts, tables = s, ts, trial
+                        slot = cand
     set_tuning(0, table_text(tables[0]))
     set_tuning(1, table_text(tables[1]))
     return {"device": device_info(), "workloads": [w.name for w in workloads],
