@@ -267,6 +267,22 @@ __device__ __forceinline__ void lds_fill(int* p, int v, int lr) {
 // lane are register compare-exchanges; the others exchange with lane
 // lr ^ (j / E) through cross-lane shuffles, E of them in flight at once.
 
+// The value of lane lr ^ LJ (LJ < L <= 64, groups of L aligned lanes):
+// a DPP quad permutation for distances 1 and 2, a swizzle (xor within 32
+// lanes, no address operand) up to 16, a general cross-lane read for 32.
+template <int LJ, int L>
+__device__ __forceinline__ int lane_xor(int v) {
+    static_assert(LJ >= 1 && LJ < L && L <= kWave, "bad lane distance");
+    if constexpr (LJ == 1)
+        return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    else if constexpr (LJ == 2)
+        return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    else if constexpr (LJ < 32)
+        return __builtin_amdgcn_ds_swizzle(v, (LJ << 10) | 0x1F);
+    else
+        return __shfl_xor(v, LJ, L);
+}
+
 // One stage (level k, distance J) of the network.
 template <int E, int L, int J>
 __device__ __forceinline__ void bitonic_step_regs(int (&x)[E], int lr, int base, int k) {
@@ -275,7 +291,7 @@ __device__ __forceinline__ void bitonic_step_regs(int (&x)[E], int lr, int base,
         const bool lower = (lr & LJ) == 0;
         int other[E];
 #pragma unroll
-        for (int s = 0; s < E; ++s) other[s] = __shfl_xor(x[s], LJ, L);
+        for (int s = 0; s < E; ++s) other[s] = lane_xor<LJ, L>(x[s]);
 #pragma unroll
         for (int s = 0; s < E; ++s) {
             const bool asc = ((base + lr * E + s) & k) == 0;

@@ -109,6 +109,20 @@ inline T __shfl_xor(T v, int mask, int width = 64) {
     idx = idx >= ((self + width) & ~(width - 1)) ? self : idx;
     return emu_shfl_any(v, idx);
 }
+// DPP quad permutation (the two patterns the library uses: lane ^ 1 and
+// lane ^ 2 within each quad) and the swizzle bit mode (lane ^ xor within
+// each group of 32: pattern = xor << 10 | and_mask 0x1f).
+inline int __builtin_amdgcn_mov_dpp(int v, int ctrl, int, int, bool) {
+    const int self = emu::lane();
+    const int sel = (ctrl >> (2 * (self & 3))) & 3;
+    return emu_shfl_any(v, (self & ~3) | sel);
+}
+inline int __builtin_amdgcn_ds_swizzle(int v, int pattern) {
+    const int self = emu::lane();
+    const int and_mask = pattern & 0x1f, or_mask = (pattern >> 5) & 0x1f, xor_mask = (pattern >> 10) & 0x1f;
+    const int j = ((self & and_mask) | or_mask) ^ xor_mask;
+    return emu_shfl_any(v, (self & ~31) | j);
+}
 inline unsigned long long __ballot(int pred) {
     emu::exchange_write(pred ? 1u : 0u);
     emu::wave_barrier();
