@@ -236,6 +236,14 @@ def run_batched(n_inst: int, reps: int) -> dict:
     batched(), loop()  # warm the pools
     tb = min(batched() for _ in range(reps))
     tl = min(loop() for _ in range(reps))
+    # the same batch with graph replay on: call 1 eager, call 2 captures, then replays
+    g.set_graphs(True)
+    try:
+        batched(), batched()
+        tg = min(batched() for _ in range(reps))
+        gstats = g.graph_stats()
+    finally:
+        g.set_graphs(False)
     for a, b in dev:
         a.free()
         b.free()
@@ -244,6 +252,7 @@ def run_batched(n_inst: int, reps: int) -> dict:
                        "rmat_rows": "1/3", "seed": 3}, "dtype": "f64", "nip": int(nip),
             "batched_ms": round(tb * 1e3, 3), "loop_ms": round(tl * 1e3, 3),
             "batched_products_per_s": nip / tb, "loop_products_per_s": nip / tl, "speedup": tl / tb,
+            "graph_replay_ms": round(tg * 1e3, 3), "graph_stats": gstats,
             "timing": f"best of {reps}, wall clock around the whole call (outputs allocated inside)"}
 
 

@@ -27,7 +27,7 @@ from .csr import DeviceCSR, HostCSR, _dtype_code
 __all__ = [
     "HostCSR", "DeviceCSR", "SpgError", "proposal", "proposal_batched", "get_nip",
     "device_info", "set_tuning", "get_tuning", "get_variants", "variant_list", "profile", "load_library",
-    "Server",
+    "Server", "set_graphs", "graph_stats", "graphs",
 ]
 
 
@@ -117,6 +117,31 @@ def variant_list(phase: int) -> list[dict]:
             v["lanes"] = v["wg"] // v["rows"]
         out.append(v)
     return out
+
+
+def set_graphs(enabled: bool) -> None:
+    """Graph replay of repeated calls (spg_set_graphs): the second call
+    with the same launch signature captures each phase into a HIP graph,
+    later ones replay it."""
+    _lib.check(_lib.load().spg_set_graphs(1 if enabled else 0))
+
+
+def graph_stats() -> dict:
+    """Graphs captured and phase replays since the context was created."""
+    cap, rep = ctypes.c_int64(), ctypes.c_int64()
+    _lib.check(_lib.load().spg_graph_stats(ctypes.byref(cap), ctypes.byref(rep)))
+    return {"captured": cap.value, "replayed": rep.value}
+
+
+class graphs:
+    """Context manager turning graph replay on: `with graphs(): ...`."""
+
+    def __enter__(self):
+        set_graphs(True)
+        return self
+
+    def __exit__(self, *exc):
+        set_graphs(False)
 
 
 class profile:

@@ -66,6 +66,8 @@ _SIG = {
     "spg_server_destroy": (_I32, [_P]),
     "spg_get_tuning": (_I32, [_I32, ctypes.c_char_p, ctypes.c_size_t]),
     "spg_get_variants": (_I32, [_I32, ctypes.c_char_p, ctypes.c_size_t]),
+    "spg_set_graphs": (_I32, [_I32]),
+    "spg_graph_stats": (_I32, [ctypes.POINTER(ctypes.c_int64)] * 2),
     "spg_set_profiling": (_I32, [_I32]),
     "spg_profile_count": (_I32, []),
     "spg_profile_get": (_I32, [_I32, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int64),

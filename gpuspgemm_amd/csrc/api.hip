@@ -18,6 +18,7 @@
 #include "../../include/gpuspgemm.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <chrono>
 #include <condition_variable>
@@ -72,6 +73,63 @@ struct ProfRec {
     double ms;
 };
 
+// Instantiated graphs of one phase of a multiply, keyed by everything that
+// fixes the phase's launches: sizes, bin counts, tables and the addresses
+// of the lane's buffers.  A signature is captured the second time it is
+// seen (a product shape run once never pays for a capture); the cache keeps
+// the most recently used graphs.
+struct GraphCache {
+    static constexpr size_t kEntries = 64;
+    static constexpr size_t kSeen = 256;
+    struct Entry {
+        std::vector<int64_t> sig;
+        hipGraphExec_t exec = nullptr;
+        uint64_t used = 0;
+    };
+    // a fixed array: the handles stay in place for the life of the process
+    Entry entries[kEntries];
+    size_t n = 0;
+    std::deque<std::vector<int64_t>> seen;
+    uint64_t clock = 0;
+
+    hipGraphExec_t find(const std::vector<int64_t>& sig) {
+        for (size_t i = 0; i < n; ++i)
+            if (entries[i].sig == sig) {
+                entries[i].used = ++clock;
+                return entries[i].exec;
+            }
+        return nullptr;
+    }
+    // true when `sig` was seen before; records it otherwise
+    bool seen_before(const std::vector<int64_t>& sig) {
+        if (std::find(seen.begin(), seen.end(), sig) != seen.end()) return true;
+        seen.push_back(sig);
+        if (seen.size() > kSeen) seen.pop_front();
+        return false;
+    }
+    void insert(std::vector<int64_t> sig, hipGraphExec_t exec) {
+        size_t at = n;
+        if (n == kEntries) {  // replace the least recently used
+            at = 0;
+            for (size_t i = 1; i < n; ++i)
+                if (entries[i].used < entries[at].used) at = i;
+            (void)hipGraphExecDestroy(entries[at].exec);
+        } else {
+            ++n;
+        }
+        seen.erase(std::remove(seen.begin(), seen.end(), sig), seen.end());
+        entries[at] = Entry{std::move(sig), exec, ++clock};
+    }
+    void clear() {
+        for (size_t i = 0; i < n; ++i) {
+            (void)hipGraphExecDestroy(entries[i].exec);
+            entries[i] = Entry{};
+        }
+        n = 0;
+        seen.clear();
+    }
+};
+
 // The device resources of one multiply in flight: a main stream plus aux
 // streams that the fused-pass launches are spread over, events, a scratch
 // arena that only grows, and pinned host memory for the read-backs.  The
@@ -85,13 +143,15 @@ struct Lane {
     hipEvent_t ev_ready = nullptr;
     hipEvent_t ev_done[kStreams - 1] = {};
     hipEvent_t ev_host = nullptr;  // error check while the host polls
-    int* h_flag = nullptr;          // pinned: completion sequence stored by k_signal
-    int seq = 0;
+    int* h_flag = nullptr;          // pinned: cleared by the host, set to 1 by k_signal
     char* scratch = nullptr;
     size_t scratch_cap = 0;
+    char* gpool = nullptr;  // global-bin tables of the fused pass under graph replay
+    size_t gpool_cap = 0;
     DevMeta* h_meta = nullptr;
     char* h_stage = nullptr;
     size_t h_stage_cap = 0;
+    GraphCache graphs[2];  // symbolic phase, fused pass
 };
 
 struct Context {
@@ -109,6 +169,8 @@ struct Context {
     int occ_sym_multi[2] = {1, 1}, occ_sym_max = 1;
     int occ_num[2][kNumLdsVariants] = {};
     Lane lane;
+    bool graphs = false;  // replay repeated phases from captured graphs
+    std::atomic<int64_t> n_captured{0}, n_replayed{0};
     bool profiling = false;
     std::vector<ProfRec> prof;
     std::vector<hipEvent_t> events;
@@ -203,34 +265,45 @@ void lane_destroy(Lane& L) {
     }
     if (L.ev_ready) (void)hipEventDestroy(L.ev_ready);
     if (L.ev_host) (void)hipEventDestroy(L.ev_host);
+    L.graphs[0].clear();
+    L.graphs[1].clear();
     if (L.h_flag) (void)hipHostFree(L.h_flag);
     if (L.scratch) (void)hipFree(L.scratch);
+    if (L.gpool) (void)hipFree(L.gpool);
     if (L.h_meta) (void)hipHostFree(L.h_meta);
     if (L.h_stage) (void)hipHostFree(L.h_stage);
     if (L.s) (void)hipStreamDestroy(L.s);
     L = Lane{};
 }
 
-// Wait for everything queued on the lane's main stream.  A kernel queued
-// last stores a new sequence number into pinned host memory and the host
-// polls that word: the runtime's own completion of an event or stream was
-// measured at ~55 us after the GPU finished, at each of the multiply's two
-// read-backs.  The event is queried now and then so that a failed stream
-// ends the wait with its error.
-hipError_t lane_wait(Lane& L) {
-    const int seq = ++L.seq;
-    launch_signal(L.h_flag, seq, L.s);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipEventRecord(L.ev_host, L.s);
+// Completion of the work queued on a lane's main stream.  The host clears a
+// word of pinned host memory (lane_arm), a kernel queued last stores 1 into
+// it (launch_signal, eagerly or as a graph's last node) and the host polls
+// the word (lane_poll): the runtime's own completion of an event or stream
+// was measured at ~55 us after the GPU finished, at each of the multiply's
+// two read-backs.  The event is queried now and then so that a failed stream
+// ends the wait with its error.  Every signal is polled for before the lane
+// is armed again, so no earlier store can land after the clear.
+void lane_arm(Lane& L) { __atomic_store_n(L.h_flag, 0, __ATOMIC_RELEASE); }
+
+hipError_t lane_poll(Lane& L) {
+    hipError_t e = hipEventRecord(L.ev_host, L.s);
     if (e != hipSuccess) return e;
     for (unsigned spin = 1;; ++spin) {
-        if (__atomic_load_n(L.h_flag, __ATOMIC_ACQUIRE) == seq) return hipSuccess;
+        if (__atomic_load_n(L.h_flag, __ATOMIC_ACQUIRE) == 1) return hipSuccess;
         if ((spin & 1023) == 0) {
             e = hipEventQuery(L.ev_host);
-            if (e == hipSuccess) return __atomic_load_n(L.h_flag, __ATOMIC_ACQUIRE) == seq ? hipSuccess : hipErrorUnknown;
+            if (e == hipSuccess) return __atomic_load_n(L.h_flag, __ATOMIC_ACQUIRE) == 1 ? hipSuccess : hipErrorUnknown;
             if (e != hipErrorNotReady) return e;
         }
     }
+}
+
+hipError_t lane_wait(Lane& L) {
+    lane_arm(L);
+    launch_signal(L.h_flag, L.s);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? lane_poll(L) : e;
 }
 
 spg_status ensure_context(Context*& out) {
@@ -306,6 +379,18 @@ spg_status ensure_scratch(Lane& L, size_t bytes) {
     L.scratch_cap = 0;
     SPG_HIP(hipMalloc(reinterpret_cast<void**>(&L.scratch), cap));
     L.scratch_cap = cap;
+    return SPG_OK;
+}
+
+spg_status ensure_gpool(Lane& L, size_t bytes) {
+    if (bytes <= L.gpool_cap) return SPG_OK;
+    SPG_HIP(hipStreamSynchronize(L.s));
+    if (L.gpool) SPG_HIP(hipFree(L.gpool));
+    L.gpool = nullptr;
+    const size_t cap = std::max(bytes, L.gpool_cap * 3 / 2);
+    L.gpool_cap = 0;
+    SPG_HIP(hipMalloc(reinterpret_cast<void**>(&L.gpool), cap));
+    L.gpool_cap = cap;
     return SPG_OK;
 }
 
@@ -431,15 +516,17 @@ struct Job : JobBase {
         long long nnz_total = 0;
     } hm;
     std::vector<long long> nnz;
-    void* num_pool = nullptr;
+    void* num_pool = nullptr;   // allocated per call (eager) or the lane's pool (graph replay)
+    bool graphs = false;        // phases may be replayed from the lane's graph caches
+    bool signal_queued = false; // the fused pass already queued its completion signal
 
     Job(Context& ctx, Lane& lane, const BinTable& sym_table, const BinTable& num_table, bool profile)
-        : c(ctx), L(lane), prof(ctx, profile), sym(sym_table), num(num_table) {}
+        : c(ctx), L(lane), prof(ctx, profile), sym(sym_table), num(num_table), graphs(ctx.graphs && !profile) {}
 
     void abort() override {
         for (void* p : owned) (void)dev_release(c, p, L.s);
         owned.clear();
-        if (num_pool) (void)dev_release(c, num_pool, L.s);
+        if (num_pool && !graphs) (void)dev_release(c, num_pool, L.s);
         num_pool = nullptr;
         for (int i = 0; i < count; ++i) C[i] = spg_csr{0, 0, 0, nullptr, nullptr, nullptr};
     }
@@ -499,6 +586,8 @@ struct Job : JobBase {
         return SPG_OK;
     }
 
+    // scan stage and the read-back of the counters, ending with the
+    // completion signal (the caller armed the lane and polls it)
     spg_status scan(int after_fail) {
         const hipStream_t s = L.s;
         prof.begin("scan_count", total_rows, n_tiles, s);
@@ -511,7 +600,59 @@ struct Job : JobBase {
         if (batch)
             SPG_HIP(hipMemcpyAsync(L.h_stage + stage_inst, inst_nnz, sizeof(long long) * count, hipMemcpyDeviceToHost,
                                    s));
-        SPG_HIP(lane_wait(L));
+        launch_signal(L.h_flag, s);
+        SPG_HIP(hipGetLastError());
+        return SPG_OK;
+    }
+
+    // Signature of a phase: every value its launches depend on.  Buffer
+    // contents (operand descriptors, tiles) are copied from pinned host
+    // memory by the phase itself and so are not part of it.
+    std::vector<int64_t> signature(int phase) const {
+        std::vector<int64_t> k{phase, static_cast<int64_t>(sizeof(V)), count, n_tiles, total_rows,
+                               reinterpret_cast<int64_t>(L.scratch), reinterpret_cast<int64_t>(L.h_stage)};
+        const BinTable& t = phase == 0 ? sym : num;
+        k.push_back(t.n);
+        for (int b = 0; b < t.n; ++b) {
+            k.push_back(t.variant[b]);
+            k.push_back(t.nip_cap[b]);
+            k.push_back(t.nnz_cap[b]);
+            if (phase == 1) k.push_back(hm.num_count[b]);
+        }
+        if (phase == 1) k.push_back(reinterpret_cast<int64_t>(num_pool));
+        return k;
+    }
+
+    // Queue one phase.  `record` queues its work on the lane's streams,
+    // ending with the completion signal.  With graph replay on, a phase
+    // whose signature was seen before is captured into a graph once and
+    // replayed from then on: one launch in place of a dozen.
+    template <typename F>
+    spg_status queue_phase(int phase, F&& record) {
+        lane_arm(L);
+        if (!graphs) return record();
+        std::vector<int64_t> sig = signature(phase);
+        GraphCache& gc = L.graphs[phase];
+        hipGraphExec_t exec = gc.find(sig);
+        if (exec) {
+            ++c.n_replayed;
+        } else {
+            if (!gc.seen_before(sig)) return record();
+            SPG_HIP(hipStreamBeginCapture(L.s, hipStreamCaptureModeRelaxed));
+            spg_status st = record();
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(L.s, &g);
+            if (st == SPG_OK && e != hipSuccess) st = hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+            if (st == SPG_OK) {
+                const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+                if (ei != hipSuccess) st = hip_fail(ei, "hipGraphInstantiate", __FILE__, __LINE__);
+            }
+            if (g) (void)hipGraphDestroy(g);
+            SPG_TRY(st);
+            gc.insert(std::move(sig), exec);
+            ++c.n_captured;
+        }
+        SPG_HIP(hipGraphLaunch(exec, L.s));
         return SPG_OK;
     }
 
@@ -539,6 +680,8 @@ struct Job : JobBase {
             }
         }
         nnz.assign(count, 0);
+        // graph replay reads the descriptors from device memory in every call
+        batch = count > 1 || graphs;
         if (total_rows >= INT_MAX) return failed(fail(SPG_ERR_OVERFLOW, "total row count exceeds int32"));
         if (n_tiles == 0) {  // every product has zero rows
             empty = true;
@@ -620,8 +763,14 @@ struct Job : JobBase {
             std::memcpy(L.h_stage + stage_tiles, tiles.data(), sizeof(Tile) * n_tiles);
         }
 
-        spg_status st = front();
-        if (st == SPG_OK) st = scan(0);
+        spg_status st = queue_phase(0, [&]() -> spg_status {
+            SPG_TRY(front());
+            return scan(0);
+        });
+        if (st == SPG_OK) {
+            const hipError_t e = lane_poll(L);
+            if (e != hipSuccess) st = hip_fail(e, "symbolic phase", __FILE__, __LINE__);
+        }
         if (st == SPG_OK && L.h_meta->fail_count.v > 0) {
             // rows that overflowed the largest LDS table: count them again in
             // global memory, then redo the scan stage
@@ -633,7 +782,12 @@ struct Job : JobBase {
                 launch_sym_global<V>(ps, fail_items, fail_off, static_cast<int*>(sym_pool), fails, s);
                 prof.end();
                 launch_clear(meta, kMetaScanWord, s);
+                lane_arm(L);
                 st = scan(1);
+                if (st == SPG_OK) {
+                    const hipError_t e = lane_poll(L);
+                    if (e != hipSuccess) st = hip_fail(e, "symbolic phase", __FILE__, __LINE__);
+                }
             }
             if (sym_pool) (void)dev_release(c, sym_pool, s);
         }
@@ -673,12 +827,7 @@ struct Job : JobBase {
             probs[i].c_val = static_cast<V*>(pv);
         }
         ps.p0 = probs[0];
-        if (batch) {
-            std::memcpy(L.h_stage + stage_probs, probs.data(), sizeof(Prob<V>) * count);
-            const hipError_t e =
-                hipMemcpyAsync(d_probs, L.h_stage + stage_probs, sizeof(Prob<V>) * count, hipMemcpyHostToDevice, s);
-            if (e != hipSuccess) return failed(hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__));
-        }
+        if (batch) std::memcpy(L.h_stage + stage_probs, probs.data(), sizeof(Prob<V>) * count);
 
         // fused pass: one launch per non-empty bin, heaviest bins first,
         // spread over the lane's streams so that small bins overlap
@@ -723,49 +872,67 @@ struct Job : JobBase {
         }
         auto stream_of = [&](int q) { return q == 0 ? s : L.aux[q - 1]; };
         if (glob_bin >= 0) {
-            const spg_status st = dev_alloc(c, &num_pool, sizeof(int32_t) * hm.gnum_pool, s);
+            const size_t pool_bytes = sizeof(int32_t) * hm.gnum_pool;
+            spg_status st = SPG_OK;
+            if (graphs) {  // a buffer of the lane, so that its address recurs
+                st = ensure_gpool(L, pool_bytes);
+                num_pool = L.gpool;
+            } else {
+                st = dev_alloc(c, &num_pool, pool_bytes, s);
+            }
             if (st != SPG_OK) return failed(st);
         }
-        hipError_t e = hipSuccess;
-        if (n_streams > 1) {  // C.col, C.val and the pool were allocated on the main stream
-            e = hipEventRecord(L.ev_ready, s);
-            for (int i = 1; i < n_streams && e == hipSuccess; ++i) e = hipStreamWaitEvent(L.aux[i - 1], L.ev_ready, 0);
-        }
-        if (e == hipSuccess && glob_bin >= 0) {
-            const hipStream_t on = stream_of(0);
-            prof.begin("symnum_global", hm.num_count[glob_bin], hm.num_count[glob_bin], on);
-            launch_symnum_global<V>(ps, items2, gpool_off, glob_off, hm.num_count[glob_bin],
-                                    static_cast<int*>(num_pool), meta, on);
-            prof.end();
-        }
-        for (size_t li = 0; li < launches.size(); ++li) {
-            if (e != hipSuccess) break;
-            const Launch& Lh = launches[li];
-            const NumVariant& nv = kNumVariants[Lh.variant];
-            const int units = (Lh.count + nv.r - 1) / nv.r;
-            const int grid = (units + kXcds - 1) / kXcds * kXcds;  // one unit per workgroup
-            const hipStream_t on = stream_of(stream_idx[li]);
-            prof.begin("symnum_v" + std::to_string(Lh.variant), Lh.count, grid, on);
-            e = launch_symnum_bin<V>(Lh.variant, ps, items2, Lh.off, Lh.count, meta, grid, on);
-            prof.end();
-        }
-        for (int i = 1; i < n_streams && e == hipSuccess; ++i) {
-            e = hipEventRecord(L.ev_done[i - 1], L.aux[i - 1]);
-            if (e == hipSuccess) e = hipStreamWaitEvent(s, L.ev_done[i - 1], 0);
-        }
-        if (e == hipSuccess) e = hipGetLastError();
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(&L.h_meta->flags.v, &meta->flags.v, sizeof(int32_t), hipMemcpyDeviceToHost, s);
-        if (num_pool && e == hipSuccess) {
+        const spg_status st = queue_phase(1, [&]() -> spg_status {
+            hipError_t e = hipSuccess;
+            if (batch)
+                e = hipMemcpyAsync(d_probs, L.h_stage + stage_probs, sizeof(Prob<V>) * count, hipMemcpyHostToDevice, s);
+            // C.col, C.val and the pool were allocated on the main stream
+            if (e == hipSuccess && n_streams > 1) {
+                e = hipEventRecord(L.ev_ready, s);
+                for (int i = 1; i < n_streams && e == hipSuccess; ++i) e = hipStreamWaitEvent(L.aux[i - 1], L.ev_ready, 0);
+            }
+            if (e == hipSuccess && glob_bin >= 0) {
+                const hipStream_t on = stream_of(0);
+                prof.begin("symnum_global", hm.num_count[glob_bin], hm.num_count[glob_bin], on);
+                launch_symnum_global<V>(ps, items2, gpool_off, glob_off, hm.num_count[glob_bin],
+                                        static_cast<int*>(num_pool), meta, on);
+                prof.end();
+            }
+            for (size_t li = 0; li < launches.size(); ++li) {
+                if (e != hipSuccess) break;
+                const Launch& Lh = launches[li];
+                const NumVariant& nv = kNumVariants[Lh.variant];
+                const int units = (Lh.count + nv.r - 1) / nv.r;
+                const int grid = (units + kXcds - 1) / kXcds * kXcds;  // one unit per workgroup
+                const hipStream_t on = stream_of(stream_idx[li]);
+                prof.begin("symnum_v" + std::to_string(Lh.variant), Lh.count, grid, on);
+                e = launch_symnum_bin<V>(Lh.variant, ps, items2, Lh.off, Lh.count, meta, grid, on);
+                prof.end();
+            }
+            for (int i = 1; i < n_streams && e == hipSuccess; ++i) {
+                e = hipEventRecord(L.ev_done[i - 1], L.aux[i - 1]);
+                if (e == hipSuccess) e = hipStreamWaitEvent(s, L.ev_done[i - 1], 0);
+            }
+            if (e == hipSuccess) e = hipGetLastError();
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(&L.h_meta->flags.v, &meta->flags.v, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) {
+                launch_signal(L.h_flag, s);
+                e = hipGetLastError();
+            }
+            return e == hipSuccess ? SPG_OK : hip_fail(e, "fused pass", __FILE__, __LINE__);
+        });
+        if (st != SPG_OK) return failed(st);
+        signal_queued = true;
+        if (num_pool && !graphs) {
             (void)dev_release(c, num_pool, s);
             num_pool = nullptr;
         }
-        if (e != hipSuccess) return failed(hip_fail(e, "fused pass", __FILE__, __LINE__));
         return SPG_OK;
     }
 
     spg_status finish() override {
-        const hipError_t e = lane_wait(L);
+        const hipError_t e = signal_queued ? lane_poll(L) : lane_wait(L);
         if (e != hipSuccess) return failed(hip_fail(e, "fused pass", __FILE__, __LINE__));
         if (!empty && (L.h_meta->flags.v & kFlagMismatch))
             return failed(fail(SPG_ERR_INTERNAL, "fused pass found a different column count than the count pass "
@@ -1138,6 +1305,23 @@ spg_status spg_get_variants(int32_t phase, char* buf, size_t buf_len) {
         return fail(SPG_ERR_INVALID_ARGUMENT, "phase must be 0 or 1");
     }
     return write_text(out.str(), buf, buf_len);
+}
+
+spg_status spg_set_graphs(int32_t enabled) {
+    std::lock_guard<std::recursive_mutex> lock(g_mutex);
+    Context* c = nullptr;
+    SPG_TRY(ensure_context(c));
+    c->graphs = enabled != 0;
+    return SPG_OK;
+}
+
+spg_status spg_graph_stats(int64_t* captured, int64_t* replayed) {
+    std::lock_guard<std::recursive_mutex> lock(g_mutex);
+    Context* c = nullptr;
+    SPG_TRY(ensure_context(c));
+    if (captured) *captured = c->n_captured.load();
+    if (replayed) *replayed = c->n_replayed.load();
+    return SPG_OK;
 }
 
 spg_status spg_set_profiling(int32_t enabled) {

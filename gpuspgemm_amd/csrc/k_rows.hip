@@ -371,15 +371,17 @@ void launch_total_nip(const int32_t* a_rpt, const int32_t* a_col, const int32_t*
                        a_col, b_rpt, m, out);
 }
 
-// Completion signal for the host: one lane stores `seq` into pinned host
-// memory (a vector store at system scope).  Queued behind a read-back on the
-// same stream, it tells a polling host that the read-back has landed.
-__global__ __launch_bounds__(64) void k_signal(int* host_flag, int seq) {
-    if (threadIdx.x == 0) __hip_atomic_store(host_flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+// Completion signal for the host: one lane stores 1 into a word of pinned
+// host memory (a vector store at system scope) that the host cleared before
+// queueing the work.  Queued behind a read-back on the same stream, it tells
+// a polling host that the read-back has landed.  The value is a constant so
+// that the launch can be replayed from a graph.
+__global__ __launch_bounds__(64) void k_signal(int* host_flag) {
+    if (threadIdx.x == 0) __hip_atomic_store(host_flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-void launch_signal(int* host_flag, int seq, hipStream_t s) {
-    hipLaunchKernelGGL(k_signal, dim3(1), dim3(64), 0, s, host_flag, seq);
+void launch_signal(int* host_flag, hipStream_t s) {
+    hipLaunchKernelGGL(k_signal, dim3(1), dim3(64), 0, s, host_flag);
 }
 
 void launch_clear(DevMeta* meta, int from_word, hipStream_t s) {

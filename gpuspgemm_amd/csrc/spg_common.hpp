@@ -97,8 +97,9 @@ struct alignas(16) Tile {
     int32_t inst, row0, rows, first_tile;
 };
 
-// All products of one pass.  A single product travels in kernel arguments;
-// a batch is read from device memory.
+// All products of one pass.  A single product travels in kernel arguments
+// (`many` null); a batch, or any call replayed from a graph, is read from
+// device memory, so that the kernel arguments stay the same across calls.
 template <typename V>
 struct ProbSet {
     Prob<V> p0;
@@ -108,11 +109,11 @@ struct ProbSet {
     int32_t n_tiles;
 
     __device__ __forceinline__ GProb<V> get(int i) const {
-        if (n_inst == 1) return global_view<V>(p0);
+        if (many == nullptr) return global_view<V>(p0);
         return global_view<V>(to_global(many)[i]);
     }
     __device__ __forceinline__ Tile tile(int t) const {
-        if (n_inst == 1) {
+        if (tiles == nullptr) {
             const int r0 = t * kTileRows;
             return Tile{0, r0, min(kTileRows, p0.m - r0), 0};
         }

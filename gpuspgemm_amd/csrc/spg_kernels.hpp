@@ -100,7 +100,7 @@ __host__ __device__ constexpr int num_rows_per_unit(int v) { return kNumVariants
 
 // ---- launchers (defined next to their kernels) ----
 void launch_clear(DevMeta* meta, int from_word, hipStream_t s);
-void launch_signal(int* host_flag, int seq, hipStream_t s);
+void launch_signal(int* host_flag, hipStream_t s);
 void launch_total_nip(const int32_t* a_rpt, const int32_t* a_col, const int32_t* b_rpt, int m,
                       unsigned long long* out, hipStream_t s);
 

@@ -30,6 +30,7 @@
  *   spg_init, spg_device_info, spg_set_tuning, spg_get_tuning,
  *   spg_get_variants, spg_set_profiling, spg_profile_count, spg_profile_get,
  *   spg_multiply_batched (many products in one pass of the kernels),
+ *   spg_set_graphs, spg_graph_stats (graph replay of repeated calls),
  *   spg_server_* (a request loop with continuous batching).
  */
 #ifndef GPUSPGEMM_H
@@ -124,6 +125,22 @@ spg_status spg_set_profiling(int32_t enabled);
 int32_t spg_profile_count(void);
 spg_status spg_profile_get(int32_t index, char *name, size_t name_len, int64_t *rows,
                            int32_t *grid, double *ms);
+
+/* Graph replay (off by default).  When on, each of a multiply's two
+   phases (symbolic: NIP, binning, counts, scan, read-back; fused pass:
+   every bin launch over the lane's streams and its read-back) is described
+   by a signature of everything its launches depend on: the value type, the
+   number of products, their total rows and tiles, the bin tables, the
+   per-bin row counts of the fused pass and the addresses of the library's
+   own buffers.  The second call with a signature captures the phase into a
+   HIP graph; later calls with it replay the graph, one launch in place of
+   one per kernel.  Operand and output addresses may change from call to
+   call: the kernels read them from a device table that the graph refreshes
+   from pinned host memory each time.  Results are identical either way.
+   Profiling (spg_set_profiling) turns replay off while it is on. */
+spg_status spg_set_graphs(int32_t enabled);
+/* graphs captured and phase replays since the context was created */
+spg_status spg_graph_stats(int64_t *captured, int64_t *replayed);
 
 /* ---- served form ----------------------------------------------------
  * A server owns three lanes (stream sets with their own scratch), each

@@ -1,11 +1,14 @@
 // Driver of the CPU emulation build: C = A . B through the library's C ABI
 // (kernels emulated on the host), compared with the host Gustavson product.
 //   emu_spgemm A.csr B.csr [--f32] [--sym-table FILE] [--num-table FILE] [--repeat N]
-//              [--serve N BUDGET_BYTES MAX_BATCH]
+//              [--graphs] [--serve N BUDGET_BYTES MAX_BATCH]
 // Exit status 0 when the structure is identical and values agree to 1e-12
 // (FP64) / 1e-5 (FP32) after |A||B| scaling.  --serve submits the product N
 // times to a server (spg_server_*) and checks every result; a budget below
 // one output must fail every request with SPG_ERR_OUT_OF_MEMORY.
+// --graphs turns graph replay on; repeat r then multiplies a fresh device
+// copy of A with its values scaled by r + 1 (new addresses, new values,
+// same launches), and the graph counters are printed.
 #include <cstdio>
 #include <fstream>
 #include <sstream>
@@ -23,7 +26,7 @@ static std::string slurp(const std::string& f) {
 }
 
 template <typename T>
-static int run(const std::string& fa, const std::string& fb, int repeat) {
+static int run(const std::string& fa, const std::string& fb, int repeat, bool graphs) {
     const auto ha = utils::HostCSR<T>::load_from_filename(fa);
     const auto hb = utils::HostCSR<T>::load_from_filename(fb);
     const auto da = ha.template to<utils::Location::Device>();
@@ -32,12 +35,24 @@ static int run(const std::string& fa, const std::string& fb, int repeat) {
     const auto absref = host::abs_product(ha, hb);
     int bad = 0;
     for (int r = 0; r < repeat; ++r) {
-        const auto hc = proposal::proposal(da, db).template to<utils::Location::Host>();
+        auto hc = graphs ? [&] {
+            auto hs = ha;
+            for (int i = 0; i < hs.nnz; ++i) hs.val[i] *= static_cast<T>(r + 1);
+            const auto ds = hs.template to<utils::Location::Device>();
+            auto out = proposal::proposal(ds, db).template to<utils::Location::Host>();
+            for (int i = 0; i < out.nnz; ++i) out.val[i] /= static_cast<T>(r + 1);
+            return out;
+        }() : proposal::proposal(da, db).template to<utils::Location::Host>();
         const auto e = host::errors(hc, ref, absref);
         const double tol = sizeof(T) == 8 ? 1e-12 : 1e-5;
         std::printf("run %d: nnz %d (ref %d) structure %s scaled_err %.3e\n", r, hc.nnz, ref.nnz,
                     e.structure ? "identical" : "DIFFERENT", e.scaled);
         if (!e.structure || e.scaled > tol) bad = 1;
+    }
+    if (graphs) {
+        int64_t captured = 0, replayed = 0;
+        utils::handle_spg_error(spg_graph_stats(&captured, &replayed), "graph_stats");
+        std::printf("graphs captured %lld replayed %lld\n", (long long)captured, (long long)replayed);
     }
     return bad;
 }
@@ -102,7 +117,7 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "usage: %s A.csr B.csr [--f32] [--sym-table F] [--num-table F] [--repeat N]\n", argv[0]);
         return 2;
     }
-    bool f32 = false;
+    bool f32 = false, graphs = false;
     int repeat = 1, serve_n = 0, max_batch = 1;
     long long budget = 0;
     for (int i = 3; i < argc; ++i) {
@@ -112,6 +127,8 @@ int main(int argc, char** argv) {
         } else if ((a == "--sym-table" || a == "--num-table") && i + 1 < argc) {
             const std::string t = slurp(argv[++i]);
             utils::handle_spg_error(spg_set_tuning(a == "--sym-table" ? 0 : 1, t.c_str()), "set_tuning");
+        } else if (a == "--graphs") {
+            graphs = true;
         } else if (a == "--repeat" && i + 1 < argc) {
             repeat = std::atoi(argv[++i]);
         } else if (a == "--serve" && i + 3 < argc) {
@@ -124,10 +141,11 @@ int main(int argc, char** argv) {
         }
     }
     try {
+        if (graphs) utils::handle_spg_error(spg_set_graphs(1), "set_graphs");
         if (serve_n > 0)
             return f32 ? serve<float>(argv[1], argv[2], serve_n, budget, max_batch)
                        : serve<double>(argv[1], argv[2], serve_n, budget, max_batch);
-        return f32 ? run<float>(argv[1], argv[2], repeat) : run<double>(argv[1], argv[2], repeat);
+        return f32 ? run<float>(argv[1], argv[2], repeat, graphs) : run<double>(argv[1], argv[2], repeat, graphs);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return 1;

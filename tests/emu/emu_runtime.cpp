@@ -144,6 +144,11 @@ Pool& pool() {
 
 }  // namespace
 
+std::vector<Work>*& capture_target() {
+    thread_local std::vector<Work>* target = nullptr;
+    return target;
+}
+
 int lane() { return static_cast<int>(tl_thread.x % 64); }
 void block_barrier() {
     g_wave_bar[tl_thread.x / 64].park(1);

@@ -21,6 +21,7 @@
 #include <cstring>
 #include <functional>
 #include <type_traits>
+#include <vector>
 
 #define __global__
 #define __device__
@@ -59,6 +60,14 @@ void exchange_write(std::uint64_t v);      // publish this lane's value
 std::uint64_t exchange_read(int lane);     // read lane's published value (after a wave barrier)
 void launch(dim3 grid, dim3 block, const std::function<void()>& body);
 int lane();
+// Stream work (launches, copies, fills) runs at once, or is recorded while
+// the calling thread captures a graph and runs when the graph is launched.
+using Work = std::function<void()>;
+std::vector<Work>*& capture_target();
+inline void submit(Work w) {
+    if (std::vector<Work>* r = capture_target()) r->push_back(std::move(w));
+    else w();
+}
 }  // namespace emu
 
 #define threadIdx (::emu::tl_thread)
@@ -259,11 +268,48 @@ inline hipError_t hipMemcpy(void* d, const void* s, std::size_t n, hipMemcpyKind
     if (n) std::memcpy(d, s, n);
     return hipSuccess;
 }
-inline hipError_t hipMemcpyAsync(void* d, const void* s, std::size_t n, hipMemcpyKind k, hipStream_t) {
-    return hipMemcpy(d, s, n, k);
+inline hipError_t hipMemcpyAsync(void* d, const void* s, std::size_t n, hipMemcpyKind, hipStream_t) {
+    ::emu::submit([=] {
+        if (n) std::memcpy(d, s, n);
+    });
+    return hipSuccess;
 }
 inline hipError_t hipMemsetAsync(void* d, int v, std::size_t n, hipStream_t) {
-    std::memset(d, v, n);
+    ::emu::submit([=] { std::memset(d, v, n); });
+    return hipSuccess;
+}
+
+// graphs: the work recorded between begin and end of a capture, replayed in order
+struct emu_graph_exec {
+    std::vector<::emu::Work> work;
+};
+typedef std::vector<::emu::Work>* hipGraph_t;
+typedef emu_graph_exec* hipGraphExec_t;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal, hipStreamCaptureModeThreadLocal, hipStreamCaptureModeRelaxed };
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+    if (::emu::capture_target()) return hipErrorInvalidValue;
+    ::emu::capture_target() = new std::vector<::emu::Work>;
+    return hipSuccess;
+}
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) {
+    *g = ::emu::capture_target();
+    ::emu::capture_target() = nullptr;
+    return *g ? hipSuccess : hipErrorInvalidValue;
+}
+inline hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, void*, void*, std::size_t) {
+    *e = new emu_graph_exec{*g};
+    return hipSuccess;
+}
+inline hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+    for (auto& w : e->work) w();
+    return hipSuccess;
+}
+inline hipError_t hipGraphDestroy(hipGraph_t g) {
+    delete g;
+    return hipSuccess;
+}
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t e) {
+    delete e;
     return hipSuccess;
 }
 template <typename F>
@@ -272,5 +318,7 @@ inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, F, int, s
     return hipSuccess;
 }
 
-#define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...) \
-    ::emu::launch(dim3(grid), dim3(block), [&]() { kernel(__VA_ARGS__); })
+#define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...)                           \
+    ::emu::submit([=, emu_grid_ = dim3(grid), emu_block_ = dim3(block)]() {                     \
+        ::emu::launch(emu_grid_, emu_block_, [&]() { kernel(__VA_ARGS__); });                   \
+    })

@@ -89,6 +89,26 @@ def test_uniform_stencil_powerlaw(emu, tmp_path):
     run_emu(emu, tmp_path, P, P, "--repeat", "2")
 
 
+def test_graph_replay(emu, tmp_path):
+    """Graph replay on: each repeat multiplies a fresh copy of A with new
+    values (new addresses, same launches).  The first call runs eagerly, the
+    second captures both phases, the rest replay them; every result must
+    equal the host product."""
+    A = gen.powerlaw(m=1500, seed=5)
+    out = run_emu(emu, tmp_path, A, A, "--repeat", "4", "--graphs")
+    assert out.count("structure identical") == 4
+    assert "graphs captured 2 replayed 4" in out
+    A = gen.random_csr(200, 150, 0.04, seed=6, empty_rows=0.3)
+    B = gen.random_csr(150, 120, 0.05, seed=7)
+    out = run_emu(emu, tmp_path, A, B, "--repeat", "3", "--graphs")
+    assert out.count("structure identical") == 3
+    assert "graphs captured 2 replayed 2" in out
+    # the server's lanes replay too: batches of three identical requests
+    P = gen.powerlaw(m=800, seed=9)
+    out = run_emu(emu, tmp_path, P, P, "--serve", "9", str(1 << 30), "3", "--graphs")
+    assert out.count("structure identical") == 9
+
+
 def test_rectangular_and_empty_rows(emu, tmp_path):
     A = gen.random_csr(120, 90, 0.05, seed=1, empty_rows=0.3)
     B = gen.random_csr(90, 70, 0.06, seed=2, empty_rows=0.2)
