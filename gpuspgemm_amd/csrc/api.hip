@@ -2,14 +2,16 @@
 //
 // proposal_inner (src/libproposal/include/proposal/proposal.cuh:26-80)
 // creates eight streams, allocates its scratch and synchronises with the
-// host at least ten times per multiply.  Here a per-device context keeps
-// one stream, a scratch arena that only grows, pinned host memory for the
-// one read-back, and the bin tables; a multiply is
+// host at least ten times per multiply.  Here a per-device context keeps a
+// lane (four streams, a scratch arena that only grows, pinned host memory
+// for the read-backs) and the bin tables; a multiply is
 //
-//   clear, nip, scatter, sym_multi, sym_max, scan_count, scan_apply
+//   nip (with the symbolic binning), sym_multi x2 || sym_max, scan_count,
+//   scan_apply
 //   -> one device-to-host copy of the counters + one synchronisation
 //   -> allocate C.col / C.val of the exact size
-//   -> symnum_multi (+ symnum_big / symnum_global when those bins are used)
+//   -> one symnum launch per non-empty fused bin over the four streams,
+//      then a kernel that zeroes the counters and signals the host
 //
 // Rows that overflow the largest LDS table add a second round trip
 // (global tables, then the scan again), as the reference's fail bin does
@@ -146,6 +148,7 @@ struct Lane {
     int* h_flag = nullptr;          // pinned: cleared by the host, set to 1 by k_signal
     char* scratch = nullptr;
     size_t scratch_cap = 0;
+    bool meta_clean = false;  // the counters at the start of scratch are zero
     char* gpool = nullptr;  // global-bin tables of the fused pass under graph replay
     size_t gpool_cap = 0;
     DevMeta* h_meta = nullptr;
@@ -377,6 +380,7 @@ spg_status ensure_scratch(Lane& L, size_t bytes) {
     L.scratch = nullptr;
     const size_t cap = std::max(bytes, L.scratch_cap * 3 / 2);
     L.scratch_cap = 0;
+    L.meta_clean = false;
     SPG_HIP(hipMalloc(reinterpret_cast<void**>(&L.scratch), cap));
     L.scratch_cap = cap;
     return SPG_OK;
@@ -550,14 +554,17 @@ struct Job : JobBase {
             SPG_HIP(hipMemcpyAsync(d_tiles, L.h_stage + stage_tiles, sizeof(Tile) * n_tiles, hipMemcpyHostToDevice,
                                    s));
         }
-        prof.begin("clear", 0, 1, s);
-        launch_clear(meta, 0, s);
-        prof.end();
+        // The counters are cleared by the last kernel of a multiply that
+        // completed; after anything else (first use, grown scratch, an error
+        // or an abandoned job) they are cleared here.
+        if (graphs || !L.meta_clean) {
+            prof.begin("clear", 0, 1, s);
+            launch_clear(meta, 0, s);
+            prof.end();
+        }
+        L.meta_clean = false;
         prof.begin("nip", total_rows, n_tiles, s);
-        launch_nip<V>(ps, sym, meta, s);
-        prof.end();
-        prof.begin("scatter_sym", total_rows, n_tiles, s);
-        launch_scatter<V>(ps, sym, meta, items1, s);
+        launch_nip<V>(ps, sym, meta, items1, static_cast<int>(total_rows), s);
         prof.end();
         // The largest-table bin runs on an aux stream beside the multi-bin
         // launch (it is queued first: its workgroups without a row exit at
@@ -567,7 +574,8 @@ struct Job : JobBase {
         {
             const int grid = c.cus * c.occ_sym_max;
             prof.begin("sym_max", 0, grid, L.aux[0]);
-            launch_sym_max<V>(ps, items1, sym, sym.n - 1, meta, fail_items, fail_off, grid, L.aux[0]);
+            launch_sym_max<V>(ps, items1, static_cast<int>(total_rows), sym, sym.n - 1, meta, fail_items, fail_off,
+                              grid, L.aux[0]);
             prof.end();
         }
         const int n_multi = sym.n - 1;
@@ -578,7 +586,7 @@ struct Job : JobBase {
             int grid = c.cus * c.occ_sym_multi[part];
             grid = (grid + kXcds - 1) / kXcds * kXcds;
             prof.begin(part == 0 ? "sym_multi_small" : "sym_multi", total_rows, grid, s);
-            launch_sym_multi<V>(part, ps, items1, sym, n_multi, meta, grid, s);
+            launch_sym_multi<V>(part, ps, items1, static_cast<int>(total_rows), sym, n_multi, meta, grid, s);
             prof.end();
         }
         SPG_HIP(hipEventRecord(L.ev_done[0], L.aux[0]));
@@ -700,7 +708,7 @@ struct Job : JobBase {
         const size_t o_tsum = take(sizeof(long long) * n_tiles);
         const size_t o_inst = take(sizeof(long long) * count);
         const size_t o_nip = take(sizeof(int32_t) * M);
-        const size_t o_it1 = take(sizeof(int2) * M);
+        const size_t o_it1 = take(sizeof(int2) * M * sym.n);  // a region of M items per symbolic bin
         const size_t o_it2 = take(sizeof(int2) * M);
         const size_t o_fit = take(sizeof(int2) * M);
         const size_t o_foff = take(sizeof(long long) * M);
@@ -917,7 +925,7 @@ struct Job : JobBase {
             if (e == hipSuccess)
                 e = hipMemcpyAsync(&L.h_meta->flags.v, &meta->flags.v, sizeof(int32_t), hipMemcpyDeviceToHost, s);
             if (e == hipSuccess) {
-                launch_signal(L.h_flag, s);
+                launch_signal_clear(L.h_flag, meta, s);
                 e = hipGetLastError();
             }
             return e == hipSuccess ? SPG_OK : hip_fail(e, "fused pass", __FILE__, __LINE__);
@@ -937,6 +945,7 @@ struct Job : JobBase {
         if (!empty && (L.h_meta->flags.v & kFlagMismatch))
             return failed(fail(SPG_ERR_INTERNAL, "fused pass found a different column count than the count pass "
                                                  "(are B's rows free of duplicate columns and indices in range?)"));
+        if (signal_queued) L.meta_clean = true;  // k_signal_clear zeroed the counters
         owned.clear();  // the outputs now belong to the caller
         return SPG_OK;
     }

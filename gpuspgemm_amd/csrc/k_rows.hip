@@ -1,6 +1,6 @@
-// Row-parallel passes: counter reset, NIP + symbolic binning, scatter into
-// bin lists, and the two-kernel exclusive scan that turns nnz per row into
-// row pointers while binning rows for the fused pass.
+// Row-parallel passes: counter reset, NIP + symbolic binning with the
+// scatter into the bin lists, and the two-kernel exclusive scan that turns
+// nnz per row into row pointers while binning rows for the fused pass.
 //
 // One workgroup handles one tile of kTileRows rows of one product; with a
 // single product tiles are implicit, with a batch they come from a table.
@@ -17,10 +17,16 @@ __global__ __launch_bounds__(256) void k_clear(DevMeta* meta, int from_word) {
 // NIP_i = sum over k in A(i,:) of nnz(B(k,:)) (setup.cuh:48-91): a thread
 // per row, or the whole wave on one row when a row of the wave has more
 // than 32 entries.  Rows with NIP 0 get nnz 0 right away and join no bin
-// (binning.cuh:60-66).
+// (binning.cuh:60-66).  The other rows are scattered into the symbolic bin
+// lists in the same pass (binning.cuh:76-120): bin b's list is the region
+// items[b * bin_cap, (b + 1) * bin_cap) with bin_cap >= the total row
+// count, so a tile's place in it is one atomic add on the bin's count and
+// no offsets have to be known beforehand.
 template <typename V>
-__global__ __launch_bounds__(kRowThreads) void k_nip(ProbSet<V> ps, BinTable sym, DevMeta* meta) {
+__global__ __launch_bounds__(kRowThreads) void k_nip(ProbSet<V> ps, BinTable sym, DevMeta* meta,
+                                                     int2* __restrict__ items, int bin_cap) {
     __shared__ int s_cnt[kMaxBins];
+    __shared__ int s_base[kMaxBins];
     __shared__ int s_max;
     __shared__ unsigned long long s_total;
     __shared__ int s_flags;
@@ -64,6 +70,7 @@ __global__ __launch_bounds__(kRowThreads) void k_nip(ProbSet<V> ps, BinTable sym
     int my_max = 0;
     long long my_total = 0;
     int my_flags = 0;
+    int bin[kPer], rank[kPer];
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         const int r = threadIdx.x + j * kRowThreads;
@@ -99,7 +106,7 @@ __global__ __launch_bounds__(kRowThreads) void k_nip(ProbSet<V> ps, BinTable sym
             part = wave_sum64(part);
             if (lane_id() == src) s[j] = part;
         }
-        int bin = -1;
+        bin[j] = -1;
         if (valid) {
             if (s[j] > INT_MAX) {
                 my_flags |= kFlagNipOverflow;
@@ -110,11 +117,11 @@ __global__ __launch_bounds__(kRowThreads) void k_nip(ProbSet<V> ps, BinTable sym
             if (nip == 0)
                 P.c_rpt[row] = 0;
             else
-                bin = find_bin(sym, nip, 0);
+                bin[j] = find_bin(sym, nip, 0);
             my_max = max(my_max, nip);
             my_total += s[j];
         }
-        wave_count_add(s_cnt, bin);
+        rank[j] = wave_rank_add(s_cnt, bin[j]);
     }
     my_max = wave_max(my_max);
     my_total = wave_sum64(my_total);
@@ -125,49 +132,18 @@ __global__ __launch_bounds__(kRowThreads) void k_nip(ProbSet<V> ps, BinTable sym
     if (my_flags) atomicOr(&s_flags, my_flags);
     __syncthreads();
     if (threadIdx.x < sym.n && s_cnt[threadIdx.x] > 0)
-        atomicAdd(&meta->sym_count[threadIdx.x].v, s_cnt[threadIdx.x]);
+        s_base[threadIdx.x] = atomicAdd(&meta->sym_count[threadIdx.x].v, s_cnt[threadIdx.x]);
     if (threadIdx.x == 0) {
         atomicMax(&meta->max_nip.v, s_max);
         atomicAdd(&meta->total_nip.v, s_total);
         if (s_flags) atomicOr(&meta->flags.v, s_flags);
     }
-}
-
-// Scatter rows into the symbolic bin lists (binning.cuh:76-120), with the
-// bin offsets derived on the device from the final counts, so no host
-// round trip is needed.
-template <typename V>
-__global__ __launch_bounds__(kRowThreads) void k_scatter(ProbSet<V> ps, BinTable sym, DevMeta* meta,
-                                                         int2* __restrict__ items) {
-    __shared__ int s_cnt[kMaxBins];
-    __shared__ int s_base[kMaxBins];
-    if (threadIdx.x < kMaxBins) s_cnt[threadIdx.x] = 0;
-    __syncthreads();
-    const Tile t = ps.tile(blockIdx.x);
-    const GProb<V> P = ps.get(t.inst);
-    int bin[kTileRows / kRowThreads];
-    int rank[kTileRows / kRowThreads];
-#pragma unroll
-    for (int j = 0; j < kTileRows / kRowThreads; ++j) {
-        const int r = threadIdx.x + j * kRowThreads;
-        bin[j] = -1;
-        if (r < t.rows) {
-            const int nip = P.nip[t.row0 + r];
-            if (nip > 0) bin[j] = find_bin(sym, nip, 0);
-        }
-        rank[j] = wave_rank_add(s_cnt, bin[j]);
-    }
-    __syncthreads();
-    if (threadIdx.x < sym.n && s_cnt[threadIdx.x] > 0) {
-        int off = 0;
-        for (int b = 0; b < static_cast<int>(threadIdx.x); ++b) off += meta->sym_count[b].v;
-        s_base[threadIdx.x] = off + atomicAdd(&meta->sym_cursor[threadIdx.x].v, s_cnt[threadIdx.x]);
-    }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kTileRows / kRowThreads; ++j)
+    for (int j = 0; j < kPer; ++j)
         if (bin[j] >= 0)
-            items[s_base[bin[j]] + rank[j]] = make_int2(t.inst, t.row0 + threadIdx.x + j * kRowThreads);
+            items[static_cast<long long>(bin[j]) * bin_cap + s_base[bin[j]] + rank[j]] =
+                make_int2(t.inst, t.row0 + threadIdx.x + j * kRowThreads);
 }
 
 // Scan part 1: per-tile sum of nnz, NUM-bin histogram, max nnz and the
@@ -384,19 +360,29 @@ void launch_signal(int* host_flag, hipStream_t s) {
     hipLaunchKernelGGL(k_signal, dim3(1), dim3(64), 0, s, host_flag);
 }
 
+// The same signal at the end of a multiply, after zeroing the device
+// counters for the next one (the flags have been copied back before it).
+__global__ __launch_bounds__(256) void k_signal_clear(int* host_flag, DevMeta* meta) {
+    int* w = reinterpret_cast<int*>(meta);
+    constexpr int kWords = sizeof(DevMeta) / sizeof(int);
+    for (int i = threadIdx.x; i < kWords; i += 256) w[i] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(host_flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+void launch_signal_clear(int* host_flag, DevMeta* meta, hipStream_t s) {
+    hipLaunchKernelGGL(k_signal_clear, dim3(1), dim3(256), 0, s, host_flag, meta);
+}
+
 void launch_clear(DevMeta* meta, int from_word, hipStream_t s) {
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(256), 0, s, meta, from_word);
 }
 
 template <typename V>
-void launch_nip(const ProbSet<V>& ps, const BinTable& sym, DevMeta* meta, hipStream_t s) {
-    hipLaunchKernelGGL(k_nip<V>, dim3(ps.n_tiles), dim3(kRowThreads), 0, s, ps, sym, meta);
+void launch_nip(const ProbSet<V>& ps, const BinTable& sym, DevMeta* meta, int2* items, int bin_cap, hipStream_t s) {
+    hipLaunchKernelGGL(k_nip<V>, dim3(ps.n_tiles), dim3(kRowThreads), 0, s, ps, sym, meta, items, bin_cap);
 }
 
-template <typename V>
-void launch_scatter(const ProbSet<V>& ps, const BinTable& sym, DevMeta* meta, int2* items, hipStream_t s) {
-    hipLaunchKernelGGL(k_scatter<V>, dim3(ps.n_tiles), dim3(kRowThreads), 0, s, ps, sym, meta, items);
-}
 
 template <typename V>
 void launch_scan_count(const ProbSet<V>& ps, const BinTable& num, DevMeta* meta, long long* tile_sums,
@@ -413,8 +399,7 @@ void launch_scan_apply(const ProbSet<V>& ps, const BinTable& num, DevMeta* meta,
 }
 
 #define SPG_INSTANTIATE_ROWS(V)                                                                        \
-    template void launch_nip<V>(const ProbSet<V>&, const BinTable&, DevMeta*, hipStream_t);           \
-    template void launch_scatter<V>(const ProbSet<V>&, const BinTable&, DevMeta*, int2*, hipStream_t); \
+    template void launch_nip<V>(const ProbSet<V>&, const BinTable&, DevMeta*, int2*, int, hipStream_t); \
     template void launch_scan_count<V>(const ProbSet<V>&, const BinTable&, DevMeta*, long long*, int,  \
                                        hipStream_t);                                                   \
     template void launch_scan_apply<V>(const ProbSet<V>&, const BinTable&, DevMeta*, const long long*, \

@@ -54,7 +54,7 @@ __device__ __forceinline__ void sym_rows(const ProbSet<V>& ps, const int2* __res
     case v:                                                                                               \
         if constexpr (v >= LO && v < HI)                                                                  \
             sym_rows<V, kMultiThreads, kSymVariants[v].r, kSymVariants[v].t, kSymVariants[v].ec,          \
-                     kSymVariants[v].pmax>(ps, items, first, end, smem, s_tmp);                           \
+                     kSymVariants[v].pmax>(ps, bin_items, first, end, smem, s_tmp);                       \
         break;
 
 // LDS of a multi-bin launch over variants [LO, HI).
@@ -76,23 +76,20 @@ constexpr int sym_multi_lds() {
 // far smaller, so twice as many of their waves fit on a CU.
 template <typename V, int LO, int HI>
 __global__ __launch_bounds__(kMultiThreads) void k_sym_multi(ProbSet<V> ps, const int2* __restrict__ items,
-                                                             BinTable sym, int n_multi,
+                                                             int bin_cap, BinTable sym, int n_multi,
                                                              const DevMeta* __restrict__ meta) {
     static_assert(sym_multi_lds<LO, HI>() <= kSymMultiLds, "symbolic multi-bin LDS budget");
     __shared__ __align__(16) int smem[sym_multi_lds<LO, HI>() / 4];
     __shared__ int s_tmp[kMultiThreads / kWave];
     __shared__ int s_unit[kMaxBins + 1];
-    __shared__ int s_off[kMaxBins];
     __shared__ int s_cnt[kMaxBins];
     if (threadIdx.x == 0) {
-        int off = 0, units = 0;
+        int units = 0;
         for (int b = 0; b < n_multi; ++b) {
             const int c = meta->sym_count[b].v;
             const bool mine = sym.variant[b] >= LO && sym.variant[b] < HI;
-            s_off[b] = off;
             s_cnt[b] = c;
             s_unit[b] = units;
-            off += c;
             if (mine) units += (c + sym.rows[b] - 1) / sym.rows[b];
         }
         s_unit[n_multi] = units;
@@ -102,8 +99,9 @@ __global__ __launch_bounds__(kMultiThreads) void k_sym_multi(ProbSet<V> ps, cons
     for (int u = ur.first; u < ur.end; u += ur.stride) {
         int b = 0;
         while (u >= s_unit[b + 1]) ++b;
-        const int first = s_off[b] + (u - s_unit[b]) * sym.rows[b];
-        const int end = s_off[b] + s_cnt[b];
+        const int2* bin_items = items + static_cast<long long>(b) * bin_cap;  // bin b's list
+        const int first = (u - s_unit[b]) * sym.rows[b];
+        const int end = s_cnt[b];
         switch (sym.variant[b]) {
             SPG_SYM_CASE(0)
             SPG_SYM_CASE(1)
@@ -129,9 +127,9 @@ __global__ __launch_bounds__(kMultiThreads) void k_sym_multi(ProbSet<V> ps, cons
 // stops early and is queued, with its global-table slot range, for
 // k_sym_global.
 template <typename V>
-__global__ __launch_bounds__(1024) void k_sym_max(ProbSet<V> ps, const int2* __restrict__ items, int bin,
-                                                  DevMeta* __restrict__ meta, int2* __restrict__ fail_items,
-                                                  long long* __restrict__ fail_off) {
+__global__ __launch_bounds__(1024) void k_sym_max(ProbSet<V> ps, const int2* __restrict__ items, int bin_cap,
+                                                  int bin, DevMeta* __restrict__ meta,
+                                                  int2* __restrict__ fail_items, long long* __restrict__ fail_off) {
     constexpr SymVariant kV = kSymVariants[kSymMaxVariant];
     constexpr int T = kV.t;
     constexpr int WG = 1024;
@@ -143,14 +141,13 @@ __global__ __launch_bounds__(1024) void k_sym_max(ProbSet<V> ps, const int2* __r
     __shared__ Meta cm;
     __shared__ int s_tmp[WG / kWave];
     __shared__ int s_n;
-    int off = 0;
-    for (int b = 0; b < bin; ++b) off += meta->sym_count[b].v;
+    const int2* bin_items = items + static_cast<long long>(bin) * bin_cap;
     const int count = meta->sym_count[bin].v;
     for (int u = blockIdx.x; u < count; u += gridDim.x) {
         lds_fill<T, WG>(tab, kEmpty, threadIdx.x);
         if (threadIdx.x == 0) s_n = 0;
         __syncthreads();
-        const int2 it = items[off + u];
+        const int2 it = bin_items[u];
         const GProb<V> P = ps.get(it.x);
         const int a0 = P.a_rpt[it.y];
         const int a1 = P.a_rpt[it.y + 1];
@@ -232,21 +229,22 @@ __global__ __launch_bounds__(1024) void k_sym_global(ProbSet<V> ps, const int2* 
 // ----------------------------------------------------------------- launchers
 
 template <typename V>
-void launch_sym_multi(int part, const ProbSet<V>& ps, const int2* items, const BinTable& sym, int n_multi,
-                      const DevMeta* meta, int grid, hipStream_t s) {
+void launch_sym_multi(int part, const ProbSet<V>& ps, const int2* items, int bin_cap, const BinTable& sym,
+                      int n_multi, const DevMeta* meta, int grid, hipStream_t s) {
     if (part == 0)
         hipLaunchKernelGGL((k_sym_multi<V, 0, kSymSmallVariants>), dim3(grid), dim3(kMultiThreads), 0, s, ps, items,
-                           sym, n_multi, meta);
+                           bin_cap, sym, n_multi, meta);
     else
         hipLaunchKernelGGL((k_sym_multi<V, kSymSmallVariants, kSymMultiVariants>), dim3(grid), dim3(kMultiThreads), 0,
-                           s, ps, items, sym, n_multi, meta);
+                           s, ps, items, bin_cap, sym, n_multi, meta);
 }
 
 template <typename V>
-void launch_sym_max(const ProbSet<V>& ps, const int2* items, const BinTable& sym, int bin, DevMeta* meta,
-                    int2* fail_items, long long* fail_off, int grid, hipStream_t s) {
+void launch_sym_max(const ProbSet<V>& ps, const int2* items, int bin_cap, const BinTable& sym, int bin,
+                    DevMeta* meta, int2* fail_items, long long* fail_off, int grid, hipStream_t s) {
     (void)sym;
-    hipLaunchKernelGGL(k_sym_max<V>, dim3(grid), dim3(1024), 0, s, ps, items, bin, meta, fail_items, fail_off);
+    hipLaunchKernelGGL(k_sym_max<V>, dim3(grid), dim3(1024), 0, s, ps, items, bin_cap, bin, meta, fail_items,
+                       fail_off);
 }
 
 template <typename V>
@@ -275,10 +273,10 @@ int occupancy_sym_max() {
 }
 
 #define SPG_INSTANTIATE_SYM(V)                                                                           \
-    template void launch_sym_multi<V>(int, const ProbSet<V>&, const int2*, const BinTable&, int,           \
+    template void launch_sym_multi<V>(int, const ProbSet<V>&, const int2*, int, const BinTable&, int,      \
                                       const DevMeta*, int, hipStream_t);                                  \
-    template void launch_sym_max<V>(const ProbSet<V>&, const int2*, const BinTable&, int, DevMeta*, int2*,  \
-                                    long long*, int, hipStream_t);                                        \
+    template void launch_sym_max<V>(const ProbSet<V>&, const int2*, int, const BinTable&, int, DevMeta*,   \
+                                    int2*, long long*, int, hipStream_t);                                 \
     template void launch_sym_global<V>(const ProbSet<V>&, const int2*, const long long*, int*, int,        \
                                        hipStream_t);                                                      \
     template int occupancy_sym_multi<V>(int);                                                             \

@@ -2,9 +2,10 @@
 //
 // Pipeline of one multiply (reference phases in brackets, paths relative to
 // src/libproposal/include/proposal/):
-//   k_clear      zero the device counters                  [setup.cuh:146-150]
-//   k_nip        NIP per row, SYM bin histogram, max NIP   [setup.cuh:29-95, binning.cuh:39-72]
-//   k_scatter    rows -> SYM bin lists                     [binning.cuh:76-120]
+//   k_clear      zero the device counters (first use and   [setup.cuh:146-150]
+//                after errors; a completed multiply leaves them zeroed)
+//   k_nip        NIP per row, max NIP, rows -> SYM bin     [setup.cuh:29-95, binning.cuh:39-120]
+//                lists (one region per bin)
 //   k_sym_multi  exact nnz per row by LDS hash tables      [sym1.cuh:28-167]
 //   k_sym_max    largest LDS table with overflow detection [sym1.cuh:169-245]
 //   k_sym_global global-memory tables for overflowed rows  [sym1.cuh:247-306]
@@ -157,7 +158,6 @@ struct alignas(128) Counter {
 // when overflowed rows force that stage to run twice.
 struct DevMeta {
     Counter<int32_t> sym_count[kMaxBins];
-    Counter<int32_t> sym_cursor[kMaxBins];
     Counter<int32_t> fail_count;
     Counter<int32_t> flags;
     Counter<int32_t> max_nip;

@@ -101,13 +101,14 @@ __host__ __device__ constexpr int num_rows_per_unit(int v) { return kNumVariants
 // ---- launchers (defined next to their kernels) ----
 void launch_clear(DevMeta* meta, int from_word, hipStream_t s);
 void launch_signal(int* host_flag, hipStream_t s);
+void launch_signal_clear(int* host_flag, DevMeta* meta, hipStream_t s);
 void launch_total_nip(const int32_t* a_rpt, const int32_t* a_col, const int32_t* b_rpt, int m,
                       unsigned long long* out, hipStream_t s);
 
+// NIP per row and the scatter into the symbolic bin lists (bin b's list
+// starts at items + b * bin_cap)
 template <typename V>
-void launch_nip(const ProbSet<V>& ps, const BinTable& sym, DevMeta* meta, hipStream_t s);
-template <typename V>
-void launch_scatter(const ProbSet<V>& ps, const BinTable& sym, DevMeta* meta, int2* items, hipStream_t s);
+void launch_nip(const ProbSet<V>& ps, const BinTable& sym, DevMeta* meta, int2* items, int bin_cap, hipStream_t s);
 template <typename V>
 void launch_scan_count(const ProbSet<V>& ps, const BinTable& num, DevMeta* meta, long long* tile_sums,
                        int after_fail, hipStream_t s);
@@ -118,11 +119,11 @@ void launch_scan_apply(const ProbSet<V>& ps, const BinTable& num, DevMeta* meta,
 // symbolic count: multi-bin launch over bins [0, n_multi) of `sym`
 // part 0: variants [0, kSymSmallVariants); part 1: the other multi-bin variants
 template <typename V>
-void launch_sym_multi(int part, const ProbSet<V>& ps, const int2* items, const BinTable& sym, int n_multi,
-                      const DevMeta* meta, int grid, hipStream_t s);
+void launch_sym_multi(int part, const ProbSet<V>& ps, const int2* items, int bin_cap, const BinTable& sym,
+                      int n_multi, const DevMeta* meta, int grid, hipStream_t s);
 template <typename V>
-void launch_sym_max(const ProbSet<V>& ps, const int2* items, const BinTable& sym, int bin, DevMeta* meta,
-                    int2* fail_items, long long* fail_off, int grid, hipStream_t s);
+void launch_sym_max(const ProbSet<V>& ps, const int2* items, int bin_cap, const BinTable& sym, int bin,
+                    DevMeta* meta, int2* fail_items, long long* fail_off, int grid, hipStream_t s);
 template <typename V>
 void launch_sym_global(const ProbSet<V>& ps, const int2* fail_items, const long long* fail_off, int* pool,
                        int count, hipStream_t s);
