@@ -524,8 +524,15 @@ struct Job : JobBase {
     bool graphs = false;        // phases may be replayed from the lane's graph caches
     bool signal_queued = false; // the fused pass already queued its completion signal
 
+    // Only the context's own lane captures graphs: its calls run under the
+    // library mutex, which the library's synchronous helpers (spg_memcpy,
+    // spg_malloc, spg_free, spg_device_sync) take too.  HIP rejects a
+    // synchronous call on the legacy stream from any thread while some
+    // stream of the device is capturing, so a server's worker threads,
+    // which run beside other threads, never capture.
     Job(Context& ctx, Lane& lane, const BinTable& sym_table, const BinTable& num_table, bool profile)
-        : c(ctx), L(lane), prof(ctx, profile), sym(sym_table), num(num_table), graphs(ctx.graphs && !profile) {}
+        : c(ctx), L(lane), prof(ctx, profile), sym(sym_table), num(num_table),
+          graphs(ctx.graphs && !profile && &lane == &ctx.lane) {}
 
     void abort() override {
         for (void* p : owned) (void)dev_release(c, p, L.s);
@@ -1189,11 +1196,13 @@ spg_status spg_malloc(void** ptr, size_t bytes) {
     if (!ptr) return fail(SPG_ERR_INVALID_ARGUMENT, "null argument");
     *ptr = nullptr;
     if (bytes == 0) return SPG_OK;
+    std::lock_guard<std::recursive_mutex> lock(g_mutex);  // never during a graph capture
     SPG_HIP(hipMalloc(ptr, bytes));
     return SPG_OK;
 }
 
 spg_status spg_free(void* ptr) {
+    std::lock_guard<std::recursive_mutex> lock(g_mutex);
     if (ptr) SPG_HIP(hipFree(ptr));
     return SPG_OK;
 }
@@ -1201,11 +1210,13 @@ spg_status spg_free(void* ptr) {
 spg_status spg_memcpy(void* dst, const void* src, size_t bytes) {
     if (bytes == 0) return SPG_OK;
     if (!dst || !src) return fail(SPG_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard<std::recursive_mutex> lock(g_mutex);
     SPG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
     return SPG_OK;
 }
 
 spg_status spg_device_sync(void) {
+    std::lock_guard<std::recursive_mutex> lock(g_mutex);
     SPG_HIP(hipDeviceSynchronize());
     return SPG_OK;
 }

@@ -153,12 +153,15 @@ __global__ __launch_bounds__(1024) void k_sym_max(ProbSet<V> ps, const int2* __r
         const int a1 = P.a_rpt[it.y + 1];
         bool over = false;
         for (int ebase = a0; ebase < a1 && !over; ebase += EC) {
-            const int total = load_chunk<V, WG, EC, false>(P, ebase, a1, threadIdx.x, &cm, s_tmp);
+            int start[EC / WG];
+            const int total = load_chunk<V, WG, EC, false>(P, ebase, a1, threadIdx.x, &cm, s_tmp, start);
+            int prior = 0;
             for (int base = 0; base < total && !over; base += WG * PMAX) {
                 int key[PMAX];
                 V prod[PMAX];
                 bool ok[PMAX];
-                fetch_products<V, WG, EC, PMAX, false>(P, &cm, total, base, threadIdx.x, key, prod, ok);
+                fetch_products<V, WG, EC, PMAX, false>(P, &cm, total, base, threadIdx.x, start, prior, key, prod,
+                                                       ok);
                 const int mine = table_insert_all<T>(tab, key, ok);
                 if (mine) atomicAdd(&s_n, mine);
                 __syncthreads();

@@ -120,8 +120,10 @@ __device__ __forceinline__ void symnum_rows(const ProbSet<V>& ps, const int2* __
             total = load_chunk_lane<V, EC, true>(Pr, a0, a1, rm);
             fetch_products_lane<V, EC, PMAX, true>(Pr, rm, total, 0, key, prod, ok);
         } else {
-            total = load_chunk<V, L, EC, true>(Pr, a0, a1, lr, cm, s_tmp);
-            fetch_products<V, L, EC, PMAX, true>(Pr, cm, total, 0, lr, key, prod, ok);
+            int start[EC / L];
+            int prior = 0;
+            total = load_chunk<V, L, EC, true>(Pr, a0, a1, lr, cm, s_tmp, start);
+            fetch_products<V, L, EC, PMAX, true>(Pr, cm, total, 0, lr, start, prior, key, prod, ok);
         }
         table_insert_all<T>(tab, key, ok);
     } else {

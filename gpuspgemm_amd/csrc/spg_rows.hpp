@@ -5,11 +5,17 @@
 // group is the whole workgroup).  Its products a_ik * b_kj are enumerated
 // flat: the group loads the metadata of EC entries of A's row at once (B
 // row start and length, a_ik), takes an exclusive prefix of the B-row
-// lengths, and lane lr handles products p = lr, lr + L, ...; product p
-// belongs to the entry e with off[e] <= p < off[e+1], found by a binary
-// search in LDS.  Consecutive lanes read consecutive entries of a B row
-// (coalesced), every lane has up to PMAX independent loads in flight, and
-// short B rows leave no lane idle.
+// lengths, and lane lr handles products p = lr, lr + L, ...  Consecutive
+// lanes read consecutive entries of a B row (coalesced), every lane has up
+// to PMAX independent loads in flight, and short B rows leave no lane idle.
+// Product p belongs to the last entry that starts at or before p.  A group
+// of one whole wave (64 lanes) finds it by counting: per round of 64 * PMAX
+// products it sets one bit per entry start in an LDS bitmap, one 64-bit
+// word per slot, and the entry of p is the number of set bits up to p
+// (entries with empty B rows are dropped from the chunk first).  Narrower
+// and wider groups binary-search the entries' product offsets (measured:
+// for narrow groups, whose chunks are short, the counting costs more
+// registers than the search saves).
 #pragma once
 
 #include "spg_common.hpp"
@@ -77,22 +83,30 @@ __device__ __forceinline__ int group_excl_scan(int v, int lr, int* s_tmp, int& t
 // Metadata of one chunk of EC entries of A's row, per row group (in LDS for
 // L > 1, in registers for a lane that owns its row).
 template <typename V, int EC, bool VALS>
-struct ChunkMeta {
-    int off[EC];             // exclusive prefix of B-row lengths (entries past the chunk: the chunk total)
-    int bs[EC];              // first position of the B row
+struct alignas(8) ChunkMeta {
+    // L != 64: exclusive prefix of B-row lengths, entries past the chunk
+    // holding the chunk total.  L == 64: the bitmap of entry starts of the
+    // current round (one 64-bit word per slot).
+    int off[EC];
+    // position in B of the entry's first product; L == 64: minus the
+    // entry's product offset
+    int d[EC];
     V av[VALS ? EC : 1];     // a_ik
 };
 
 // Load entries [ebase, min(ebase + EC, a1)) of A's row into `cm`; returns
-// the chunk's product count.  Each lane loads EC/L consecutive entries.
+// the chunk's product count.  Each lane loads EC/L consecutive entries.  A
+// group of one whole wave keeps only entries with a non-empty B row, in
+// order, and each lane gets the product offsets where its own entries
+// start (start[t], -1 for an empty one).
 template <typename V, int L, int EC, bool VALS>
 __device__ __forceinline__ int load_chunk(const GProb<V>& P, int ebase, int a1, int lr,
-                                          ChunkMeta<V, EC, VALS>* cm, int* s_tmp) {
+                                          ChunkMeta<V, EC, VALS>* cm, int* s_tmp, int (&start)[EC / L]) {
     static_assert(EC % L == 0, "chunk must be a multiple of the group");
     constexpr int PER = EC / L;
     int len[PER], bs[PER];
     V av[PER];
-    int local = 0;
+    int local = 0, filled = 0;
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
         const int e = ebase + lr * PER + t;
@@ -106,16 +120,34 @@ __device__ __forceinline__ int load_chunk(const GProb<V>& P, int ebase, int a1, 
             len[t] = P.b_rpt[k + 1] - bs[t];
         }
         local += len[t];
+        filled += len[t] > 0 ? 1 : 0;
     }
     int total;
     int run = group_excl_scan<L>(local, lr, s_tmp, total);
+    if constexpr (L == kWave) {
+        int n_filled;
+        int k = group_excl_scan<L>(filled, lr, s_tmp, n_filled);
 #pragma unroll
-    for (int t = 0; t < PER; ++t) {
-        const int e = lr * PER + t;
-        cm->off[e] = run;
-        cm->bs[e] = bs[t];
-        if constexpr (VALS) cm->av[e] = av[t];
-        run += len[t];
+        for (int t = 0; t < PER; ++t) {
+            start[t] = -1;
+            if (len[t] > 0) {
+                cm->d[k] = bs[t] - run;
+                if constexpr (VALS) cm->av[k] = av[t];
+                start[t] = run;
+                ++k;
+            }
+            run += len[t];
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < PER; ++t) {
+            const int e = lr * PER + t;
+            cm->off[e] = run;
+            cm->d[e] = bs[t];
+            if constexpr (VALS) cm->av[e] = av[t];
+            start[t] = run;
+            run += len[t];
+        }
     }
     gsync<L>();
     return total;
@@ -132,22 +164,60 @@ __device__ __forceinline__ int chunk_find(const int* off, int p) {
 }
 
 // Fetch up to PMAX products of a chunk: slot j holds product
-// base + j*L + lr.  ok[j] tells which slots are real.
+// base + j*L + lr.  ok[j] tells which slots are real.  `prior` counts the
+// entries that start before `base` (0 for a chunk's first round) and is
+// advanced past this round.
 template <typename V, int L, int EC, int PMAX, bool VALS>
-__device__ __forceinline__ void fetch_products(const GProb<V>& P, const ChunkMeta<V, EC, VALS>* cm, int total,
-                                               int base, int lr, int (&key)[PMAX], V (&prod)[PMAX],
-                                               bool (&ok)[PMAX]) {
+__device__ __forceinline__ void fetch_products(const GProb<V>& P, ChunkMeta<V, EC, VALS>* cm, int total, int base,
+                                               int lr, const int (&start)[EC / L], int& prior, int (&key)[PMAX],
+                                               V (&prod)[PMAX], bool (&ok)[PMAX]) {
+    if constexpr (L == kWave) {
+        // slot j covers bitmap word j; lane lr counts the set bits at or
+        // below its own position.  (Moving the word to scalar registers and
+        // counting with mbcnt measured slower.)
+        static_assert(kWave == 64, "one bitmap word per slot");
+        static_assert(2 * PMAX <= EC, "the bitmap lives in the chunk's offset array");
+        unsigned long long* bits = reinterpret_cast<unsigned long long*>(cm->off);
+        if (lr < PMAX) bits[lr] = 0;
+        wave_sync();
 #pragma unroll
-    for (int j = 0; j < PMAX; ++j) {
-        const int p = base + j * L + lr;
-        ok[j] = p < total;
-        key[j] = 0;
-        prod[j] = V(0);
-        if (ok[j]) {
-            const int e = chunk_find<EC>(cm->off, p);
-            const int q = cm->bs[e] + p - cm->off[e];
-            key[j] = P.b_col[q];
-            if constexpr (VALS) prod[j] = cm->av[e] * P.b_val[q];
+        for (int t = 0; t < EC / L; ++t) {
+            const int q = start[t] - base;
+            if (start[t] >= 0 && q >= 0 && q < L * PMAX) atomicOr(&bits[q >> 6], 1ull << (q & 63));
+        }
+        wave_sync();
+        const unsigned long long upto = ~0ull >> (63 - lr);  // positions 0..lr
+        int cnt = prior;  // entries starting before slot j's window
+#pragma unroll
+        for (int j = 0; j < PMAX; ++j) {
+            const unsigned long long w = bits[j];
+            const int p = base + j * L + lr;
+            ok[j] = p < total;
+            key[j] = 0;
+            prod[j] = V(0);
+            if (ok[j]) {
+                const int e = cnt + __popcll(w & upto) - 1;
+                const int at = p + cm->d[e];
+                key[j] = P.b_col[at];
+                if constexpr (VALS) prod[j] = cm->av[e] * P.b_val[at];
+            }
+            cnt += __popcll(w);
+        }
+        prior = cnt;
+        wave_sync();  // the bitmap is rewritten by the next round
+    } else {
+#pragma unroll
+        for (int j = 0; j < PMAX; ++j) {
+            const int p = base + j * L + lr;
+            ok[j] = p < total;
+            key[j] = 0;
+            prod[j] = V(0);
+            if (ok[j]) {
+                const int e = chunk_find<EC>(cm->off, p);
+                const int at = cm->d[e] + p - cm->off[e];
+                key[j] = P.b_col[at];
+                if constexpr (VALS) prod[j] = cm->av[e] * P.b_val[at];
+            }
         }
     }
 }
@@ -173,7 +243,7 @@ __device__ __forceinline__ int load_chunk_lane(const GProb<V>& P, int ebase, int
 #pragma unroll
     for (int t = 0; t < EC; ++t) {
         cm.off[t] = run;
-        cm.bs[t] = bs[t];
+        cm.d[t] = bs[t];
         run += len[t];
     }
     return run;
@@ -189,13 +259,13 @@ __device__ __forceinline__ void fetch_products_lane(const GProb<V>& P, const Chu
         key[j] = 0;
         prod[j] = V(0);
         if (ok[j]) {
-            int off = cm.off[0], bs = cm.bs[0];
+            int off = cm.off[0], bs = cm.d[0];
             V av = VALS ? cm.av[0] : V(0);
 #pragma unroll
             for (int t = 1; t < EC; ++t) {
                 if (cm.off[t] <= p) {
                     off = cm.off[t];
-                    bs = cm.bs[t];
+                    bs = cm.d[t];
                     if constexpr (VALS) av = cm.av[t];
                 }
             }
@@ -223,9 +293,11 @@ __device__ __forceinline__ void for_each_round(const GProb<V>& P, int a0, int a1
                 g(key, prod, ok);
             }
         } else {
-            const int total = load_chunk<V, L, EC, VALS>(P, ebase, a1, lr, cm, s_tmp);
+            int start[EC / L];
+            const int total = load_chunk<V, L, EC, VALS>(P, ebase, a1, lr, cm, s_tmp, start);
+            int prior = 0;
             for (int base = 0; base < total; base += L * PMAX) {
-                fetch_products<V, L, EC, PMAX, VALS>(P, cm, total, base, lr, key, prod, ok);
+                fetch_products<V, L, EC, PMAX, VALS>(P, cm, total, base, lr, start, prior, key, prod, ok);
                 g(key, prod, ok);
             }
             gsync<L>();  // the chunk metadata is overwritten next

@@ -137,7 +137,12 @@ spg_status spg_profile_get(int32_t index, char *name, size_t name_len, int64_t *
    one per kernel.  Operand and output addresses may change from call to
    call: the kernels read them from a device table that the graph refreshes
    from pinned host memory each time.  Results are identical either way.
-   Profiling (spg_set_profiling) turns replay off while it is on. */
+   Profiling (spg_set_profiling) turns replay off while it is on.  Replay
+   applies to spg_multiply / spg_multiply_batched; a server's lanes run
+   eagerly.  While a call captures, HIP rejects synchronous calls on the
+   legacy stream from every thread of the process: the library's own
+   helpers wait for the call, other code must not run such calls beside
+   a multiply with replay on. */
 spg_status spg_set_graphs(int32_t enabled);
 /* graphs captured and phase replays since the context was created */
 spg_status spg_graph_stats(int64_t *captured, int64_t *replayed);

@@ -141,6 +141,22 @@ inline unsigned long long __ballot(int pred) {
     emu::wave_barrier();
     return m;
 }
+// value of the first active lane of the wave
+inline unsigned __builtin_amdgcn_readfirstlane(unsigned v) {
+    const unsigned long long active = __ballot(1);
+    return static_cast<unsigned>(emu_shfl_any(static_cast<int>(v), __builtin_ctzll(active)));
+}
+// acc + the set bits of `mask` at lane positions below this lane (low / high 32 lanes)
+inline unsigned __builtin_amdgcn_mbcnt_lo(unsigned mask, unsigned acc) {
+    const int self = emu::lane();
+    const unsigned below = self >= 32 ? 0xffffffffu : ((1u << self) - 1u);
+    return acc + static_cast<unsigned>(__builtin_popcount(mask & below));
+}
+inline unsigned __builtin_amdgcn_mbcnt_hi(unsigned mask, unsigned acc) {
+    const int self = emu::lane();
+    const unsigned below = self < 32 ? 0u : ((1u << (self - 32)) - 1u);
+    return acc + static_cast<unsigned>(__builtin_popcount(mask & below));
+}
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 inline int __ffsll(long long x) { return __builtin_ffsll(x); }
 
@@ -185,6 +201,9 @@ inline int atomicMax(int* p, int v) {
     return old;
 }
 inline int atomicOr(int* p, int v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
+inline unsigned long long atomicOr(unsigned long long* p, unsigned long long v) {
+    return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST);
+}
 
 // ---- runtime
 typedef int hipError_t;

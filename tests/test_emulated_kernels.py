@@ -103,7 +103,7 @@ def test_graph_replay(emu, tmp_path):
     out = run_emu(emu, tmp_path, A, B, "--repeat", "3", "--graphs")
     assert out.count("structure identical") == 3
     assert "graphs captured 2 replayed 2" in out
-    # the server's lanes replay too: batches of three identical requests
+    # the server's lanes run eagerly with replay on
     P = gen.powerlaw(m=800, seed=9)
     out = run_emu(emu, tmp_path, P, P, "--serve", "9", str(1 << 30), "3", "--graphs")
     assert out.count("structure identical") == 9
@@ -133,6 +133,34 @@ def test_fused_variant(emu, tmp_path, variant):
     m = rows_for(lanes)
     run_emu(emu, tmp_path, *squares(m, int(np.sqrt(p)), seed=10 + variant), tables=table)
     run_emu(emu, tmp_path, *chunked(m, min(p, 2 * ec + 1), seed=10 + variant), "--f32", tables=table)
+
+
+WAVE_ROWS = [(0, v) for v, (lanes, _, _) in sorted(SYM.items()) if lanes == 64] + \
+            [(1, v) for v, (lanes, _, _) in sorted(NUM.items()) if lanes == 64]
+
+
+@pytest.mark.parametrize("phase,variant", WAVE_ROWS)
+def test_wave_rows_count_entry_starts(emu, tmp_path, phase, variant):
+    """Rows of one full wave find each product's A entry by counting entry
+    starts in a bitmap: chunks whose products span more than one round
+    (symbolic: rows of ~60 entries and NIP near the table size; fused: rows
+    of ~120 entries, two chunks, ~2000 products), with B rows of uneven
+    length and 40 % of them empty (dropped from the chunk before counting)."""
+    if phase == 0:
+        _, _, t = SYM[variant]
+        table = {0: f"{variant} {t - 1} {INT_MAX}\n{SYM_MAX} {INT_MAX} {INT_MAX}\n"}
+        A = gen.random_csr(8, 400, 0.15, seed=60 + variant)
+        ncol = 2000
+        per_row = 0.8 * (t - 1) / (60 * 0.6)
+    else:
+        _, _, p = NUM[variant]
+        table = {1: f"{variant} {INT_MAX} {p}\n{NUM_GLOBAL} {INT_MAX} {INT_MAX}\n"}
+        A = gen.random_csr(8, 400, 0.3, seed=60 + variant)
+        ncol = min(p, 400)
+        per_row = 2.5 * 64 * 12 / (120 * 0.6)
+    B = gen.random_csr(400, ncol, min(0.9, per_row / ncol), seed=70 + variant, empty_rows=0.4)
+    run_emu(emu, tmp_path, A, B, tables=table)
+    run_emu(emu, tmp_path, A.astype(np.float32), B.astype(np.float32), "--f32", tables=table)
 
 
 def test_global_paths(emu, tmp_path):

@@ -107,13 +107,15 @@ def test_batched_replays(graphs_on):
     assert after["replayed"] - before["replayed"] >= 4
 
 
-def test_server_with_replay(graphs_on):
-    """Served form with replay on: batches of identical requests repeat
-    their signatures; every result must match the oracle."""
+def test_server_runs_eagerly_with_replay_on(graphs_on):
+    """A server's lanes never capture (their worker threads run beside the
+    caller's synchronous copies); with replay on, served results stay
+    correct and no graph is captured or replayed for them."""
     A = gen.powerlaw(m=20_000, seed=25)
     ref = oracle.spgemm(A, A)
     dA = A.to_device()
     try:
+        before = g.graph_stats()
         with g.Server(budget_bytes=1 << 30, max_batch=3) as srv:
             for _ in range(3):
                 ts = [srv.submit(dA, dA) for _ in range(9)]
@@ -122,5 +124,6 @@ def test_server_with_replay(graphs_on):
                     h = dC.to_host()
                     dC.free()
                     compare(A, A, h, ref)
+        assert g.graph_stats() == before
     finally:
         dA.free()
