@@ -384,10 +384,12 @@ def main(argv=None) -> int:
     ap.add_argument("--form", default="all", choices=["all", "plain", "batched", "served"],
                     help="all: the plain line with the batched and served forms attached")
     ap.add_argument("--instances", type=int, default=512, help="batched form: products per batch")
-    ap.add_argument("--requests", type=int, default=160, help="served form: requests")
+    ap.add_argument("--requests", type=int, default=400, help="served form: requests")
     ap.add_argument("--load", type=float, default=1.0, help="served form: offered load / serial rate")
     ap.add_argument("--sat-load", type=float, default=4.0,
                     help="served form, throughput run: offered load / serial rate")
+    ap.add_argument("--mod-load", type=float, default=0.5,
+                    help="served form, latency run at a moderate load: offered load / serial rate")
     ap.add_argument("--budget-gb", type=float, default=16.0, help="served form: output memory budget")
     ap.add_argument("--max-batch", type=int, default=32, help="served form: largest batch")
     args = ap.parse_args(argv)
@@ -434,6 +436,7 @@ def main(argv=None) -> int:
     if args.form == "all":
         line["batched"] = run_batched(args.instances, 3)
         line["served"] = run_served(args.requests, args.load, args.budget_gb, args.max_batch)
+        line["served_moderate"] = run_served(args.requests, args.mod_load, args.budget_gb, args.max_batch)
         line["served_saturated"] = run_served(args.requests, args.sat_load, args.budget_gb, args.max_batch)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(A.astype(dtype), out["nip"], args.cpu_budget)
