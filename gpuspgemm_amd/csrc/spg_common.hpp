@@ -173,19 +173,6 @@ struct DevMeta {
 };
 constexpr int kMetaScanWord = static_cast<int>(offsetof(DevMeta, num_count) / sizeof(int32_t));
 
-// Launch description of a multi-bin kernel: the bins it serves, their item
-// ranges and the cumulative count of work units (one unit = one workgroup's
-// rows).  For the symbolic pass counts are read on the device instead.
-struct MultiArgs {
-    int32_t n;                  // bins served
-    int32_t bin[kMaxBins];      // bin index in the phase table
-    int32_t variant[kMaxBins];  // compiled variant
-    int32_t rows[kMaxBins];     // rows per unit
-    int32_t item_off[kMaxBins];
-    int32_t count[kMaxBins];
-    int32_t unit_start[kMaxBins + 1];
-};
-
 // ---------------------------------------------------------------- device helpers
 
 // Compile-time loop: f(std::integral_constant<int, I>) for I in [B, E).
@@ -309,17 +296,6 @@ __device__ __forceinline__ int wave_rank_add(int* cnt, int bin) {
         todo &= ~m;
     }
     return rank;
-}
-
-// Branch-free lower bound in a sorted LDS list of P entries (P a power of
-// two, padded with INT_MAX).  The key is known to be present.
-template <int P>
-__device__ __forceinline__ int lower_bound_pow2(const int* cols, int key) {
-    int pos = 0;
-#pragma unroll
-    for (int step = P >> 1; step > 0; step >>= 1)
-        if (cols[pos + step - 1] < key) pos += step;
-    return pos;
 }
 
 // Lower bounds of keys[J0, J0 + N), searched in lockstep: each halving step

@@ -1,9 +1,9 @@
 // Compiled kernel variants and host-side launchers.
 //
-// A variant fixes how a workgroup works on rows: R rows per workgroup,
-// G lanes per entry of A's row (those G lanes stride one row of B), and the
-// LDS table size.  The runtime bin tables (BinTable) map rows to variants;
-// they replace the reference's hand-set per-arch tables
+// A variant fixes how a workgroup works on rows: R rows per workgroup (so
+// L = WG / R lanes per row), A entries per chunk, products per lane per
+// round, and the LDS table size.  The runtime bin tables (BinTable) map
+// rows to variants; they replace the reference's hand-set per-arch tables
 // (src/libproposal/include/proposal/parameters.cuh:22-413) and can be
 // rewritten at run time by the tuner.
 #pragma once
@@ -94,9 +94,6 @@ __host__ __device__ __forceinline__ long long global_table_slots(long long keys)
     while (t < 2 * keys) t <<= 1;
     return t;
 }
-
-__host__ __device__ constexpr int sym_rows_per_unit(int v) { return kSymVariants[v].r; }
-__host__ __device__ constexpr int num_rows_per_unit(int v) { return kNumVariants[v].r; }
 
 // ---- launchers (defined next to their kernels) ----
 void launch_clear(DevMeta* meta, int from_word, hipStream_t s);
