@@ -504,7 +504,7 @@ struct Job : JobBase {
     long long* tile_sums = nullptr;
     long long* inst_nnz = nullptr;
     int2* items1 = nullptr;
-    int2* items2 = nullptr;
+    RowItem* items2 = nullptr;
     int2* fail_items = nullptr;
     long long* fail_off = nullptr;
     long long* gpool_off = nullptr;
@@ -716,7 +716,7 @@ struct Job : JobBase {
         const size_t o_inst = take(sizeof(long long) * count);
         const size_t o_nip = take(sizeof(int32_t) * M);
         const size_t o_it1 = take(sizeof(int2) * M * sym.n);  // a region of M items per symbolic bin
-        const size_t o_it2 = take(sizeof(int2) * M);
+        const size_t o_it2 = take(sizeof(RowItem) * M);
         const size_t o_fit = take(sizeof(int2) * M);
         const size_t o_foff = take(sizeof(long long) * M);
         const size_t o_goff = take(sizeof(long long) * M);
@@ -732,7 +732,7 @@ struct Job : JobBase {
         inst_nnz = reinterpret_cast<long long*>(base + o_inst);
         int32_t* nip = reinterpret_cast<int32_t*>(base + o_nip);
         items1 = reinterpret_cast<int2*>(base + o_it1);
-        items2 = reinterpret_cast<int2*>(base + o_it2);
+        items2 = reinterpret_cast<RowItem*>(base + o_it2);
         fail_items = reinterpret_cast<int2*>(base + o_fit);
         fail_off = reinterpret_cast<long long*>(base + o_foff);
         gpool_off = reinterpret_cast<long long*>(base + o_goff);
@@ -825,7 +825,6 @@ struct Job : JobBase {
     spg_status launch_numeric() override {
         if (empty) return SPG_OK;
         const hipStream_t s = L.s;
-        const int di = sizeof(V) == 8 ? 1 : 0;
         // exact-size outputs
         for (int i = 0; i < count; ++i) {
             if (C[i].m == 0) continue;

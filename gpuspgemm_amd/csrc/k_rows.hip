@@ -211,7 +211,7 @@ __global__ __launch_bounds__(kRowThreads) void k_scan_count(ProbSet<V> ps, BinTa
 template <typename V>
 __global__ __launch_bounds__(kRowThreads) void k_scan_apply(ProbSet<V> ps, BinTable num, DevMeta* meta,
                                                             const long long* __restrict__ tile_sums,
-                                                            int2* __restrict__ items,
+                                                            RowItem* __restrict__ items,
                                                             long long* __restrict__ gpool_off,
                                                             long long* __restrict__ inst_nnz, int after_fail) {
     if (!after_fail && meta->fail_count.v > 0) return;
@@ -247,14 +247,19 @@ __global__ __launch_bounds__(kRowThreads) void k_scan_apply(ProbSet<V> ps, BinTa
     int v[kPer];
     int bin[kPer];
     int rank[kPer];
+    int nip[kPer];
+    int ar[kPer + 1];  // the rows' A row pointers, for the fused-pass items
     long long local = 0;
     const int r0 = threadIdx.x * kPer;
+#pragma unroll
+    for (int j = 0; j <= kPer; ++j) ar[j] = r0 + j <= t.rows ? P.a_rpt[t.row0 + r0 + j] : 0;
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         const int r = r0 + j;
         v[j] = r < t.rows ? P.c_rpt[t.row0 + r] : 0;
+        nip[j] = v[j] > 0 ? P.nip[t.row0 + r] : 0;
         local += v[j];
-        bin[j] = v[j] > 0 ? find_bin(num, P.nip[t.row0 + r], v[j]) : -1;
+        bin[j] = v[j] > 0 ? find_bin(num, nip[j], v[j]) : -1;
         rank[j] = wave_rank_add(s_cnt, bin[j]);
     }
     // block exclusive scan of `local`
@@ -272,9 +277,11 @@ __global__ __launch_bounds__(kRowThreads) void k_scan_apply(ProbSet<V> ps, BinTa
     long long run = tile_base + wave_base + incl - local;
 
     int flags = 0;
+    int c0[kPer];
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         const int r = r0 + j;
+        c0[j] = static_cast<int>(run);
         if (r < t.rows) {
             if (run > INT_MAX) flags |= kFlagNnzOverflow;
             P.c_rpt[t.row0 + r] = static_cast<int>(run);
@@ -302,7 +309,7 @@ __global__ __launch_bounds__(kRowThreads) void k_scan_apply(ProbSet<V> ps, BinTa
     for (int j = 0; j < kPer; ++j) {
         if (bin[j] < 0) continue;
         const int pos = s_base[bin[j]] + rank[j];
-        items[pos] = make_int2(t.inst, t.row0 + r0 + j);
+        items[pos] = RowItem{t.inst, t.row0 + r0 + j, ar[j], ar[j + 1], c0[j], v[j], nip[j], 0};
         if (num.variant[bin[j]] == kNumGlobalVariant)
             gpool_off[pos] = static_cast<long long>(
                 atomicAdd(&meta->gnum_cursor.v, static_cast<unsigned long long>(global_table_slots(v[j]))));
@@ -393,7 +400,7 @@ void launch_scan_count(const ProbSet<V>& ps, const BinTable& num, DevMeta* meta,
 
 template <typename V>
 void launch_scan_apply(const ProbSet<V>& ps, const BinTable& num, DevMeta* meta, const long long* tile_sums,
-                       int2* items, long long* gpool_off, long long* inst_nnz, int after_fail, hipStream_t s) {
+                       RowItem* items, long long* gpool_off, long long* inst_nnz, int after_fail, hipStream_t s) {
     hipLaunchKernelGGL(k_scan_apply<V>, dim3(ps.n_tiles), dim3(kRowThreads), 0, s, ps, num, meta, tile_sums,
                        items, gpool_off, inst_nnz, after_fail);
 }
@@ -403,7 +410,7 @@ void launch_scan_apply(const ProbSet<V>& ps, const BinTable& num, DevMeta* meta,
     template void launch_scan_count<V>(const ProbSet<V>&, const BinTable&, DevMeta*, long long*, int,  \
                                        hipStream_t);                                                   \
     template void launch_scan_apply<V>(const ProbSet<V>&, const BinTable&, DevMeta*, const long long*, \
-                                       int2*, long long*, long long*, int, hipStream_t);
+                                       RowItem*, long long*, long long*, int, hipStream_t);
 
 SPG_INSTANTIATE_ROWS(float)
 SPG_INSTANTIATE_ROWS(double)

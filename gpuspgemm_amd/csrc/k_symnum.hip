@@ -75,7 +75,7 @@ __device__ __forceinline__ int sort_row(const int* tab, int* cols, int lr, int* 
 }
 
 template <typename V, int WG, int R, int P, int EC, int PMAX>
-__device__ __forceinline__ void symnum_rows(const ProbSet<V>& ps, const int2* __restrict__ items, int first,
+__device__ __forceinline__ void symnum_rows(const ProbSet<V>& ps, const RowItem* __restrict__ items, int first,
                                             int end, int* smem, int* s_tmp, DevMeta* meta) {
     constexpr int L = WG / R;
     constexpr int T = 2 * P;
@@ -98,16 +98,9 @@ __device__ __forceinline__ void symnum_rows(const ProbSet<V>& ps, const int2* __
 
     const int idx = first + slot;
     const bool active = idx < end;
-    const int2 it = active ? items[idx] : make_int2(0, 0);
-    const GProb<V> Pr = ps.get(it.x);
-    int a0 = 0, a1 = 0, nip = 0, c0 = 0, cn = 0;
-    if (active) {
-        a0 = Pr.a_rpt[it.y];
-        a1 = Pr.a_rpt[it.y + 1];
-        nip = Pr.nip[it.y];
-        c0 = Pr.c_rpt[it.y];
-        cn = Pr.c_rpt[it.y + 1] - c0;
-    }
+    const RowItem it = active ? load_item(items, idx) : RowItem{0, 0, 0, 0, 0, 0, 0, 0};
+    const GProb<V> Pr = ps.get(it.inst);
+    const int a0 = it.a0, a1 = it.a1, nip = it.nip, c0 = it.c0, cn = it.cn;
     // products staged in registers when the whole row fits in one round
     const bool fits = a1 - a0 <= EC && nip <= L * PMAX;
     int key[PMAX];
@@ -182,7 +175,7 @@ __device__ __forceinline__ void symnum_rows(const ProbSet<V>& ps, const int2* __
 // unit of a workgroup is chosen so that consecutive units stay on one XCD.
 // A smaller grid walks the units persistently.
 template <typename V, int VV>
-__global__ __launch_bounds__(kNumVariants[VV].wg) void k_symnum_bin(ProbSet<V> ps, const int2* __restrict__ items,
+__global__ __launch_bounds__(kNumVariants[VV].wg) void k_symnum_bin(ProbSet<V> ps, const RowItem* __restrict__ items,
                                                                     int item_off, int count, DevMeta* meta) {
     constexpr NumVariant kV = kNumVariants[VV];
     __shared__ __align__(16) int smem[kV.r * num_row_ints(kV)];
@@ -215,7 +208,7 @@ __device__ __forceinline__ int lower_bound_global(Ptr cols, int n, int key) {
 // C.val (num.cuh:251-298).  The reference's O(nnz^2) rank sort
 // (sym2.cuh:279-287) becomes O(nnz * chunks * log chunk).
 template <typename V>
-__global__ __launch_bounds__(1024) void k_symnum_global(ProbSet<V> ps, const int2* __restrict__ items,
+__global__ __launch_bounds__(1024) void k_symnum_global(ProbSet<V> ps, const RowItem* __restrict__ items,
                                                         const long long* __restrict__ gpool_off, int item_off,
                                                         int* pool, DevMeta* meta) {
     constexpr int WG = 1024;
@@ -227,10 +220,10 @@ __global__ __launch_bounds__(1024) void k_symnum_global(ProbSet<V> ps, const int
     __shared__ Meta cm;
     __shared__ int s_tmp[WG / kWave];
     __shared__ int s_n;
-    const int2 it = items[item_off + blockIdx.x];
-    const GProb<V> P = ps.get(it.x);
-    const int c0 = P.c_rpt[it.y];
-    const int nnz = P.c_rpt[it.y + 1] - c0;
+    const RowItem it = load_item(items, item_off + blockIdx.x);
+    const GProb<V> P = ps.get(it.inst);
+    const int c0 = it.c0;
+    const int nnz = it.cn;
     const long long slots = global_table_slots(nnz);
     const int bits = __builtin_ctzll(static_cast<unsigned long long>(slots));
     const uint32_t mask = static_cast<uint32_t>(slots - 1);
@@ -243,8 +236,8 @@ __global__ __launch_bounds__(1024) void k_symnum_global(ProbSet<V> ps, const int
     __threadfence();
     __syncthreads();
 
-    const int a0 = P.a_rpt[it.y];
-    const int a1 = P.a_rpt[it.y + 1];
+    const int a0 = it.a0;
+    const int a1 = it.a1;
     for_each_product<V, WG, EC, PMAX, false>(
         P, a0, a1, threadIdx.x, reinterpret_cast<ChunkMeta<V, EC, false>*>(&cm), s_tmp, [&](int key, V) {
             uint32_t h = hash_slot_rt(key, bits);
@@ -312,7 +305,7 @@ __global__ __launch_bounds__(1024) void k_symnum_global(ProbSet<V> ps, const int
 // ----------------------------------------------------------------- launchers
 
 template <typename V, int VV>
-static hipError_t launch_variant(const ProbSet<V>& ps, const int2* items, int item_off, int count, DevMeta* meta,
+static hipError_t launch_variant(const ProbSet<V>& ps, const RowItem* items, int item_off, int count, DevMeta* meta,
                                  int grid, hipStream_t s) {
     constexpr auto kernel = k_symnum_bin<V, VV>;
     hipLaunchKernelGGL(kernel, dim3(grid), dim3(kNumVariants[VV].wg), 0, s, ps, items, item_off, count, meta);
@@ -331,7 +324,7 @@ static int occupancy_variant() {
     X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18)
 
 template <typename V>
-hipError_t launch_symnum_bin(int variant, const ProbSet<V>& ps, const int2* items, int item_off, int count,
+hipError_t launch_symnum_bin(int variant, const ProbSet<V>& ps, const RowItem* items, int item_off, int count,
                              DevMeta* meta, int grid, hipStream_t s) {
 #define SPG_LAUNCH_CASE(v) \
     case v:                \
@@ -345,7 +338,7 @@ hipError_t launch_symnum_bin(int variant, const ProbSet<V>& ps, const int2* item
 }
 
 template <typename V>
-void launch_symnum_global(const ProbSet<V>& ps, const int2* items, const long long* gpool_off, int item_off,
+void launch_symnum_global(const ProbSet<V>& ps, const RowItem* items, const long long* gpool_off, int item_off,
                           int count, int* pool, DevMeta* meta, hipStream_t s) {
     hipLaunchKernelGGL(k_symnum_global<V>, dim3(count), dim3(1024), 0, s, ps, items, gpool_off, item_off, pool,
                        meta);
@@ -365,9 +358,9 @@ int occupancy_symnum_bin(int variant) {
 }
 
 #define SPG_INSTANTIATE_NUM(V)                                                                               \
-    template hipError_t launch_symnum_bin<V>(int, const ProbSet<V>&, const int2*, int, int, DevMeta*, int,       \
+    template hipError_t launch_symnum_bin<V>(int, const ProbSet<V>&, const RowItem*, int, int, DevMeta*, int,    \
                                              hipStream_t);                                                    \
-    template void launch_symnum_global<V>(const ProbSet<V>&, const int2*, const long long*, int, int, int*,    \
+    template void launch_symnum_global<V>(const ProbSet<V>&, const RowItem*, const long long*, int, int, int*, \
                                           DevMeta*, hipStream_t);                                             \
     template int occupancy_symnum_bin<V>(int);
 

@@ -124,6 +124,23 @@ struct ProbSet {
     }
 };
 
+// A row of the fused pass as the scan stage hands it over: its place and
+// the values a kernel reads first, so that the kernel's first dependent
+// load is the row's entries of A.
+struct alignas(16) RowItem {
+    int32_t inst, row;
+    int32_t a0, a1;  // the row's entries of A
+    int32_t c0, cn;  // its first position in C and its nnz
+    int32_t nip, pad;
+};
+
+__device__ __forceinline__ RowItem load_item(const RowItem* items, long long i) {
+    typedef int32_t Words __attribute__((vector_size(16)));
+    const auto w = to_global(reinterpret_cast<const Words*>(items + i));
+    const Words x = w[0], y = w[1];
+    return RowItem{x[0], x[1], x[2], x[3], y[0], y[1], y[2], y[3]};
+}
+
 // Runtime bin table of one phase: row -> first bin whose caps admit it.
 struct BinTable {
     int32_t n;

@@ -111,7 +111,7 @@ void launch_scan_count(const ProbSet<V>& ps, const BinTable& num, DevMeta* meta,
                        int after_fail, hipStream_t s);
 template <typename V>
 void launch_scan_apply(const ProbSet<V>& ps, const BinTable& num, DevMeta* meta, const long long* tile_sums,
-                       int2* items, long long* gpool_off, long long* inst_nnz, int after_fail, hipStream_t s);
+                       RowItem* items, long long* gpool_off, long long* inst_nnz, int after_fail, hipStream_t s);
 
 // symbolic count: multi-bin launch over bins [0, n_multi) of `sym`
 // part 0: variants [0, kSymSmallVariants); part 1: the other multi-bin variants
@@ -127,10 +127,10 @@ void launch_sym_global(const ProbSet<V>& ps, const int2* fail_items, const long 
 
 // fused pass: one launch per variant (0..9) over one bin's items
 template <typename V>
-hipError_t launch_symnum_bin(int variant, const ProbSet<V>& ps, const int2* items, int item_off, int count,
+hipError_t launch_symnum_bin(int variant, const ProbSet<V>& ps, const RowItem* items, int item_off, int count,
                              DevMeta* meta, int grid, hipStream_t s);
 template <typename V>
-void launch_symnum_global(const ProbSet<V>& ps, const int2* items, const long long* gpool_off, int item_off,
+void launch_symnum_global(const ProbSet<V>& ps, const RowItem* items, const long long* gpool_off, int item_off,
                           int count, int* pool, DevMeta* meta, hipStream_t s);
 
 // occupancy (workgroups per CU) of the persistent launches
