@@ -433,6 +433,15 @@ def main(argv=None) -> int:
             "roofline": sout["roofline"], "nip": sout["nip"], "nnz_c": sout["nnz_c"],
         }
         del S
+    if not args.no_secondary and args.workload == "powerlaw" and args.dtype == "f64":
+        # the power-law product in FP32 as well (same matrix, values rounded)
+        fres = run_gpu(A, args.steps, min(args.warmup, 3), args.pause, np.float32)
+        fout = summarize(A, fres, "f32", "powerlaw", cfg)
+        line["powerlaw_f32"] = {
+            "value": round(fout["value"], 3), "unit": "GFLOP/s", "dtype": "f32",
+            "ms_per_step": round(fout["ms_per_step"], 4), "hbm_gbps": round(fout["hbm_gbps"], 1),
+            "hbm_frac": round(fout["hbm_frac"], 4), "roofline": fout["roofline"],
+        }
     if args.form == "all":
         line["batched"] = run_batched(args.instances, 3)
         line["served"] = run_served(args.requests, args.load, args.budget_gb, args.max_batch)
