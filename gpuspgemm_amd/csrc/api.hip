@@ -532,7 +532,7 @@ struct Job : JobBase {
     // which run beside other threads, never capture.
     Job(Context& ctx, Lane& lane, const BinTable& sym_table, const BinTable& num_table, bool profile)
         : c(ctx), L(lane), prof(ctx, profile), sym(sym_table), num(num_table),
-          graphs(ctx.graphs && !profile && &lane == &ctx.lane) {}
+          graphs(&lane == &ctx.lane && ctx.graphs && !profile) {}
 
     void abort() override {
         for (void* p : owned) (void)dev_release(c, p, L.s);
