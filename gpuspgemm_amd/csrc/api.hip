@@ -1431,7 +1431,7 @@ spg_status spg_server_wait(spg_server* server, int64_t ticket, spg_csr* C, doubl
         if (it == srv->requests.end()) return fail(SPG_ERR_INVALID_ARGUMENT, "unknown ticket");
         r = it->second;
         srv->cv_done.wait(lk, [&] { return r->done; });
-        srv->requests.erase(it);
+        srv->requests.erase(ticket);  // by key: submits during the wait may have rehashed the map
     }
     *C = r->C;
     if (latency_ms) *latency_ms = 1e3 * (r->t_done - r->t_submit);
