@@ -900,11 +900,20 @@ struct Job : JobBase {
             hipError_t e = hipSuccess;
             if (batch)
                 e = hipMemcpyAsync(d_probs, L.h_stage + stage_probs, sizeof(Prob<V>) * count, hipMemcpyHostToDevice, s);
-            // C.col, C.val and the pool were allocated on the main stream
-            if (e == hipSuccess && n_streams > 1) {
-                e = hipEventRecord(L.ev_ready, s);
-                for (int i = 1; i < n_streams && e == hipSuccess; ++i) e = hipStreamWaitEvent(L.aux[i - 1], L.ev_ready, 0);
-            }
+            // C.col, C.val and the pool were allocated on the main stream: an
+            // aux stream waits for that point before its first launch.  The
+            // waits are queued just before those launches, so that the host
+            // time each one takes (several us) is not spent before the first
+            // launch of the main stream, while the GPU idles.
+            if (e == hipSuccess && n_streams > 1) e = hipEventRecord(L.ev_ready, s);
+            bool aux_ready[Lane::kStreams] = {true, false, false, false};
+            auto stream_for = [&](int q) -> hipStream_t {
+                if (!aux_ready[q]) {
+                    if (e == hipSuccess) e = hipStreamWaitEvent(L.aux[q - 1], L.ev_ready, 0);
+                    aux_ready[q] = true;
+                }
+                return stream_of(q);
+            };
             if (e == hipSuccess && glob_bin >= 0) {
                 const hipStream_t on = stream_of(0);
                 prof.begin("symnum_global", hm.num_count[glob_bin], hm.num_count[glob_bin], on);
@@ -918,7 +927,8 @@ struct Job : JobBase {
                 const NumVariant& nv = kNumVariants[Lh.variant];
                 const int units = (Lh.count + nv.r - 1) / nv.r;
                 const int grid = (units + kXcds - 1) / kXcds * kXcds;  // one unit per workgroup
-                const hipStream_t on = stream_of(stream_idx[li]);
+                const hipStream_t on = stream_for(stream_idx[li]);
+                if (e != hipSuccess) break;
                 prof.begin("symnum_v" + std::to_string(Lh.variant), Lh.count, grid, on);
                 e = launch_symnum_bin<V>(Lh.variant, ps, items2, Lh.off, Lh.count, meta, grid, on);
                 prof.end();
