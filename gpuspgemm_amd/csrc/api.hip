@@ -611,11 +611,10 @@ struct Job : JobBase {
         prof.begin("scan_apply", total_rows, n_tiles, s);
         launch_scan_apply<V>(ps, num, meta, tile_sums, items2, gpool_off, inst_nnz, after_fail, s);
         prof.end();
-        SPG_HIP(hipMemcpyAsync(L.h_meta, meta, sizeof(DevMeta), hipMemcpyDeviceToHost, s));
-        if (batch)
-            SPG_HIP(hipMemcpyAsync(L.h_stage + stage_inst, inst_nnz, sizeof(long long) * count, hipMemcpyDeviceToHost,
-                                   s));
-        launch_signal(L.h_flag, s);
+        // counters (and nnz of every product of a batch) to pinned host memory, then the signal
+        launch_readback(reinterpret_cast<const int*>(meta), reinterpret_cast<int*>(L.h_meta),
+                        static_cast<int>(sizeof(DevMeta) / sizeof(int)), inst_nnz,
+                        reinterpret_cast<long long*>(L.h_stage + stage_inst), batch ? count : 0, nullptr, L.h_flag, s);
         SPG_HIP(hipGetLastError());
         return SPG_OK;
     }
@@ -938,10 +937,9 @@ struct Job : JobBase {
                 if (e == hipSuccess) e = hipStreamWaitEvent(s, L.ev_done[i - 1], 0);
             }
             if (e == hipSuccess) e = hipGetLastError();
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(&L.h_meta->flags.v, &meta->flags.v, sizeof(int32_t), hipMemcpyDeviceToHost, s);
             if (e == hipSuccess) {
-                launch_signal_clear(L.h_flag, meta, s);
+                // the flags word back, the counters zeroed for the next multiply, the signal
+                launch_readback(&meta->flags.v, &L.h_meta->flags.v, 1, nullptr, nullptr, 0, meta, L.h_flag, s);
                 e = hipGetLastError();
             }
             return e == hipSuccess ? SPG_OK : hip_fail(e, "fused pass", __FILE__, __LINE__);
@@ -961,7 +959,7 @@ struct Job : JobBase {
         if (!empty && (L.h_meta->flags.v & kFlagMismatch))
             return failed(fail(SPG_ERR_INTERNAL, "fused pass found a different column count than the count pass "
                                                  "(are B's rows free of duplicate columns and indices in range?)"));
-        if (signal_queued) L.meta_clean = true;  // k_signal_clear zeroed the counters
+        if (signal_queued) L.meta_clean = true;  // the last read-back zeroed the counters
         owned.clear();  // the outputs now belong to the caller
         return SPG_OK;
     }

@@ -367,18 +367,30 @@ void launch_signal(int* host_flag, hipStream_t s) {
     hipLaunchKernelGGL(k_signal, dim3(1), dim3(64), 0, s, host_flag);
 }
 
-// The same signal at the end of a multiply, after zeroing the device
-// counters for the next one (the flags have been copied back before it).
-__global__ __launch_bounds__(256) void k_signal_clear(int* host_flag, DevMeta* meta) {
-    int* w = reinterpret_cast<int*>(meta);
-    constexpr int kWords = sizeof(DevMeta) / sizeof(int);
-    for (int i = threadIdx.x; i < kWords; i += 256) w[i] = 0;
+// A read-back and its completion signal in one launch: copy `words` ints
+// and `n2` long longs to pinned host memory (vector stores), optionally
+// zero the device counters for the next multiply once they are copied,
+// make the copies visible at system scope and signal.
+__global__ __launch_bounds__(256) void k_readback(const int* __restrict__ src, int* host_dst, int words,
+                                                  const long long* __restrict__ src2, long long* host_dst2, int n2,
+                                                  DevMeta* clear, int* host_flag) {
+    for (int i = threadIdx.x; i < words; i += 256) host_dst[i] = src[i];
+    for (int i = threadIdx.x; i < n2; i += 256) host_dst2[i] = src2[i];
+    if (clear) {
+        __syncthreads();  // every copy has read its word
+        int* w = reinterpret_cast<int*>(clear);
+        constexpr int kWords = sizeof(DevMeta) / sizeof(int);
+        for (int i = threadIdx.x; i < kWords; i += 256) w[i] = 0;
+    }
+    __threadfence_system();
     __syncthreads();
     if (threadIdx.x == 0) __hip_atomic_store(host_flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-void launch_signal_clear(int* host_flag, DevMeta* meta, hipStream_t s) {
-    hipLaunchKernelGGL(k_signal_clear, dim3(1), dim3(256), 0, s, host_flag, meta);
+void launch_readback(const int* src, int* host_dst, int words, const long long* src2, long long* host_dst2, int n2,
+                     DevMeta* clear, int* host_flag, hipStream_t s) {
+    hipLaunchKernelGGL(k_readback, dim3(1), dim3(256), 0, s, src, host_dst, words, src2, host_dst2, n2, clear,
+                       host_flag);
 }
 
 void launch_clear(DevMeta* meta, int from_word, hipStream_t s) {

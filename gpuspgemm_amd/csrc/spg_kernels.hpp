@@ -98,7 +98,8 @@ __host__ __device__ __forceinline__ long long global_table_slots(long long keys)
 // ---- launchers (defined next to their kernels) ----
 void launch_clear(DevMeta* meta, int from_word, hipStream_t s);
 void launch_signal(int* host_flag, hipStream_t s);
-void launch_signal_clear(int* host_flag, DevMeta* meta, hipStream_t s);
+void launch_readback(const int* src, int* host_dst, int words, const long long* src2, long long* host_dst2, int n2,
+                     DevMeta* clear, int* host_flag, hipStream_t s);
 void launch_total_nip(const int32_t* a_rpt, const int32_t* a_col, const int32_t* b_rpt, int m,
                       unsigned long long* out, hipStream_t s);
 

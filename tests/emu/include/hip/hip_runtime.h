@@ -81,6 +81,7 @@ using std::min;
 
 inline void __syncthreads() { emu::block_barrier(); }
 inline void __threadfence() { std::atomic_thread_fence(std::memory_order_seq_cst); }
+inline void __threadfence_system() { std::atomic_thread_fence(std::memory_order_seq_cst); }
 inline void __threadfence_block() { std::atomic_thread_fence(std::memory_order_seq_cst); }
 inline unsigned __lane_id() { return static_cast<unsigned>(emu::lane()); }
 
