@@ -8,10 +8,12 @@
 //
 //   nip (with the symbolic binning), sym_multi x2 || sym_max, scan_count,
 //   scan_apply
-//   -> one device-to-host copy of the counters + one synchronisation
+//   -> one kernel copies the counters to pinned host memory and signals;
+//      the host polls the signal (one synchronisation)
 //   -> allocate C.col / C.val of the exact size
 //   -> one symnum launch per non-empty fused bin over the four streams,
-//      then a kernel that zeroes the counters and signals the host
+//      then a kernel that copies the flags back, zeroes the counters and
+//      signals the host
 //
 // Rows that overflow the largest LDS table add a second round trip
 // (global tables, then the scan again), as the reference's fail bin does

@@ -170,7 +170,8 @@ struct alignas(128) Counter {
     T v;
 };
 
-// Device counters of one multiply (zeroed by k_clear).  The second group
+// Device counters of one multiply (zeroed by k_clear, or by the read-back
+// that ends the multiply before).  The second group
 // belongs to the scan / fused-bin stage and is cleared again on its own
 // when overflowed rows force that stage to run twice.
 struct DevMeta {
