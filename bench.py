@@ -397,8 +397,8 @@ def main(argv=None) -> int:
         copy_calibration(args.copy_calibration)
         return 0
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the library is single-GPU, "
-              "see DESIGN.md for how it would shard)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the library runs on one GPU; "
+              "a multi-GPU run would shard the rows of A, one process per GPU)", file=sys.stderr)
         return 2
     dtype = np.float64 if args.dtype == "f64" else np.float32
     if args.form == "batched":
