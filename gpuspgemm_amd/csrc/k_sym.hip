@@ -117,8 +117,15 @@ __global__ __launch_bounds__(kMultiThreads) void k_sym_multi(ProbSet<V> ps, cons
         }
         // Consecutive units may belong to bins with different LDS layouts,
         // and rows of L <= 64 lanes only synchronise inside their wave: no
-        // wave may start the next unit while another still uses this one.
-        __syncthreads();
+        // wave may start a unit of another layout while another wave still
+        // uses this one.  Units of the same variant give every wave the same
+        // LDS slots, so between them the waves run on without a barrier.
+        const int un = u + ur.stride;
+        if (un < ur.end) {
+            int bn = b;
+            while (un >= s_unit[bn + 1]) ++bn;
+            if (sym.variant[bn] != sym.variant[b]) __syncthreads();
+        }
     }
 }
 
