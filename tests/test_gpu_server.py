@@ -87,3 +87,27 @@ def test_admission_by_exact_output_memory(gpu_lib):
                 srv.wait(t)
     finally:
         dA.free()
+
+
+def test_plain_calls_beside_a_server(gpu_lib):
+    """Plain multiplies and copies from the calling thread while the server's
+    lanes run requests: every result of both is right, and nothing waits
+    forever on the library's lock."""
+    import oracle
+    A = gen.powerlaw(m=20_000, seed=31)
+    U = gen.uniform8(m=2048, seed=32)
+    ref_a, ref_u = oracle.spgemm(A, A), oracle.spgemm(U, U)
+    dA, dU = A.to_device(), U.to_device()
+    try:
+        with g.Server(budget_bytes=1 << 30, max_batch=4) as srv:
+            tickets = [srv.submit(dA, dA) for _ in range(12)]
+            for _ in range(6):
+                compare(U, U, _plain(dU, dU), ref_u)
+            for t in tickets:
+                dC, _ = srv.wait(t)
+                h = dC.to_host()
+                dC.free()
+                compare(A, A, h, ref_a)
+    finally:
+        dA.free()
+        dU.free()
