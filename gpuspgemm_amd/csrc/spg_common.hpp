@@ -29,7 +29,7 @@ namespace spg {
 constexpr int kWave = 64;          // CDNA wavefront width
 constexpr int kEmpty = -1;         // empty hash slot (columns are >= 0)
 constexpr int kMaxBins = 16;
-constexpr int kTileRows = 4096;    // rows per tile of the row-parallel passes
+constexpr int kTileRows = 2048;    // rows per tile of the row-parallel passes
 constexpr int kRowThreads = 1024;  // threads of the row-parallel passes (few workgroups: few atomics per counter)
 constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups dealt round-robin
 constexpr int kMultiThreads = 256; // workgroup size of the multi-bin kernels
