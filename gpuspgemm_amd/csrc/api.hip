@@ -6,8 +6,8 @@
 // lane (four streams, a scratch arena that only grows, pinned host memory
 // for the read-backs) and the bin tables; a multiply is
 //
-//   nip (with the symbolic binning), sym_multi x2 || sym_max, scan_count,
-//   scan_apply
+//   nip (with the symbolic binning), sym_multi_small || sym_multi ||
+//   sym_max (three streams), scan_count, scan_apply
 //   -> one kernel copies the counters to pinned host memory and signals;
 //      the host polls the signal (one synchronisation)
 //   -> allocate C.col / C.val of the exact size
@@ -590,16 +590,26 @@ struct Job : JobBase {
         const int n_multi = sym.n - 1;
         bool part_used[2] = {false, false};
         for (int b = 0; b < n_multi; ++b) part_used[sym.variant[b] < kSymSmallVariants ? 0 : 1] = true;
+        // With both parts in use the larger-table part runs on a second aux
+        // stream: its workgroups take the CUs that the small-table grid
+        // leaves as its last units drain.  The parts count disjoint rows.
+        const bool split = part_used[0] && part_used[1];
+        if (split) SPG_HIP(hipStreamWaitEvent(L.aux[1], L.ev_ready, 0));
         for (int part = 0; part < 2; ++part) {
             if (!part_used[part]) continue;
+            const hipStream_t where = split && part == 1 ? L.aux[1] : s;
             int grid = c.cus * c.occ_sym_multi[part];
             grid = (grid + kXcds - 1) / kXcds * kXcds;
-            prof.begin(part == 0 ? "sym_multi_small" : "sym_multi", total_rows, grid, s);
-            launch_sym_multi<V>(part, ps, items1, static_cast<int>(total_rows), sym, n_multi, meta, grid, s);
+            prof.begin(part == 0 ? "sym_multi_small" : "sym_multi", total_rows, grid, where);
+            launch_sym_multi<V>(part, ps, items1, static_cast<int>(total_rows), sym, n_multi, meta, grid, where);
             prof.end();
         }
         SPG_HIP(hipEventRecord(L.ev_done[0], L.aux[0]));
         SPG_HIP(hipStreamWaitEvent(s, L.ev_done[0], 0));
+        if (split) {
+            SPG_HIP(hipEventRecord(L.ev_done[1], L.aux[1]));
+            SPG_HIP(hipStreamWaitEvent(s, L.ev_done[1], 0));
+        }
         return SPG_OK;
     }
 
