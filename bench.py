@@ -179,19 +179,35 @@ def summarize(A, res: dict, dtype_name: str, workload: str, cfg: dict) -> dict:
     }
 
 
-def cpu_baseline(A, nip: int, budget_s: float) -> dict:
+def cpu_baseline(A, nip: int, budget_s: float, warmups: int, runs: int) -> dict:
+    """The oracle's row-parallel Gustavson timed on the host: the product
+    alone (count pass and scan, then sorted columns and values into fresh
+    arrays), all host cores and one thread, `warmups` untimed and up to
+    `runs` timed products each (fewer when one thread would pass the
+    budget), like the GPU's warmup/run protocol (common.hpp:174-204)."""
     import oracle
     threads = oracle.hw_threads()
-    oracle.spgemm(A, A, threads=threads)  # warm caches / page in
-    runs, t_total = 0, 0.0
-    while runs < 3 or (t_total < budget_s and runs < 50):
-        t0 = time.perf_counter()
-        oracle.spgemm(A, A, threads=threads)
-        t_total += time.perf_counter() - t0
-        runs += 1
-    return {"value": 2.0 * nip * runs / t_total / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": "port",
-            "sample": f"full product, {runs} runs of the oracle's row-parallel Gustavson "
-                      f"({threads} threads), {t_total:.1f} s"}
+
+    def leg(t: int, budget: float) -> dict:
+        one = float(oracle.time_product(A, A, t, 0, 1)[0])  # also pages the operands in
+        n = int(max(2, min(runs, budget / max(one, 1e-6))))
+        w = min(warmups, 1) if one * (n + warmups) > budget else warmups
+        times = oracle.time_product(A, A, t, w, n)
+        mean = float(np.mean(times))
+        return {"value": round(2.0 * nip / mean / 1e9, 4), "unit": "GFLOP/s", "cores": t, "runs": n,
+                "warmups": w, "mean_ms": round(mean * 1e3, 3),
+                "p50_ms": round(float(np.percentile(times, 50)) * 1e3, 3),
+                "p99_ms": round(float(np.percentile(times, 99)) * 1e3, 3)}
+
+    allc = leg(threads, budget_s / 3)
+    single = leg(1, 2 * budget_s / 3)
+    out = dict(allc)
+    out.update({"kind": "port",
+                "sample": f"full product, {allc['runs']} timed runs after {allc['warmups']} warm-ups of the "
+                          f"oracle's row-parallel Gustavson on {threads} threads (count pass, scan, sorted "
+                          f"columns and values into fresh arrays, FP64), and {single['runs']} on one thread",
+                "single_thread": single})
+    return out
 
 
 def _batched_pair(i: int):
@@ -258,22 +274,7 @@ def run_batched(n_inst: int, reps: int) -> dict:
 
 def _served_request(k: int) -> dict:
     from gpuspgemm_amd import generators as gen
-    r = gen.rand_unit(4, 1, np.arange(4 * k, 4 * k + 4))
-    dtype = np.float64 if r[0] < 0.7 else np.float32
-    kind = ["uniform8", "powerlaw", "stencil", "batched"][min(3, int(r[1] * 4))]
-    m = int(round(2.0 ** (10 + 10 * r[2])))
-    if kind == "uniform8":
-        A = gen.uniform8(m=m, seed=100 + k)
-        B = A
-    elif kind == "powerlaw":
-        A = gen.powerlaw(m=m, seed=100 + k)
-        B = A
-    elif kind == "stencil":
-        A = gen.stencil27(g=max(2, int(round(m ** (1 / 3)))), seed=100 + k)
-        B = A
-    else:
-        A, B = gen.batched_instance(k, seed=4, m=m)
-    return {"A": A.astype(dtype), "B": B.astype(dtype), "kind": kind, "m": A.m, "dtype": np.dtype(dtype).name}
+    return gen.served_request(k)
 
 
 def run_served(n_req: int, load: float, budget_gb: float, max_batch: int) -> dict:
@@ -448,7 +449,7 @@ def main(argv=None) -> int:
         line["served_moderate"] = run_served(args.requests, args.mod_load, args.budget_gb, args.max_batch)
         line["served_saturated"] = run_served(args.requests, args.sat_load, args.budget_gb, args.max_batch)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(A.astype(dtype), out["nip"], args.cpu_budget)
+        line["cpu_baseline"] = cpu_baseline(A.astype(dtype), out["nip"], args.cpu_budget, args.warmup, args.steps)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

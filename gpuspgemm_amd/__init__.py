@@ -27,7 +27,7 @@ from .csr import DeviceCSR, HostCSR, _dtype_code
 __all__ = [
     "HostCSR", "DeviceCSR", "SpgError", "proposal", "proposal_batched", "get_nip",
     "device_info", "set_tuning", "get_tuning", "get_variants", "variant_list", "profile", "load_library",
-    "Server", "set_graphs", "graph_stats", "graphs",
+    "Server", "set_graphs", "graph_stats", "graphs", "proposal_async", "set_log_level",
 ]
 
 
@@ -45,6 +45,27 @@ def proposal(A: DeviceCSR, B: DeviceCSR) -> DeviceCSR:
     _lib.check(lib.spg_multiply(ctypes.byref(A.raw), ctypes.byref(B.raw), ctypes.byref(out),
                                 _dtype_code(A.dtype)))
     return DeviceCSR(out, A.dtype, library_owned=True)
+
+
+def proposal_async(A: DeviceCSR, B: DeviceCSR, stream: int) -> DeviceCSR:
+    """C = A . B ordered on a caller stream (a hipStream_t handle as an int,
+    e.g. torch.cuda.Stream().cuda_stream): returns once the fused pass is
+    queued; C's contents are complete when the stream reaches the call."""
+    if A.dtype != B.dtype:
+        raise TypeError("A and B must have the same value type")
+    lib = _lib.load()
+    out = _lib.SpgCsr()
+    _lib.check(lib.spg_multiply_async(ctypes.byref(A.raw), ctypes.byref(B.raw), ctypes.byref(out),
+                                      _dtype_code(A.dtype), ctypes.c_void_p(stream)))
+    return DeviceCSR(out, A.dtype, library_owned=True)
+
+
+def set_log_level(level) -> None:
+    """Library log level: a name (trace, debug, info, warn, error, critical,
+    off) or 0..6; debug prints a line per phase of every multiply to stderr."""
+    names = ["trace", "debug", "info", "warn", "error", "critical", "off"]
+    lv = names.index(level) if isinstance(level, str) else int(level)
+    _lib.check(_lib.load().spg_set_log_level(lv))
 
 
 def proposal_batched(As: Sequence[DeviceCSR], Bs: Sequence[DeviceCSR]) -> list[DeviceCSR]:

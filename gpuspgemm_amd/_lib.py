@@ -48,7 +48,11 @@ _SIG = {
     "spg_multiply": (_I32, [ctypes.POINTER(SpgCsr), ctypes.POINTER(SpgCsr), ctypes.POINTER(SpgCsr), _I32]),
     "spg_multiply_batched": (_I32, [_I32, ctypes.POINTER(SpgCsr), ctypes.POINTER(SpgCsr),
                                     ctypes.POINTER(SpgCsr), _I32]),
+    "spg_multiply_async": (_I32, [ctypes.POINTER(SpgCsr), ctypes.POINTER(SpgCsr), ctypes.POINTER(SpgCsr), _I32,
+                                  _P]),
     "spg_csr_free": (_I32, [ctypes.POINTER(SpgCsr)]),
+    "spg_csr_free_async": (_I32, [ctypes.POINTER(SpgCsr), _P]),
+    "spg_set_log_level": (_I32, [_I32]),
     "spg_get_nip": (_I32, [ctypes.POINTER(SpgCsr), ctypes.POINTER(SpgCsr), ctypes.POINTER(ctypes.c_int64)]),
     "spg_malloc": (_I32, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]),
     "spg_free": (_I32, [_P]),

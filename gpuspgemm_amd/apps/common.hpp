@@ -124,6 +124,8 @@ int run_app(const char* name, Run run, Measure measure, int argc, char** argv) {
     Options o;
     if (const int rc = parse(argc, argv, o); rc >= 0) return rc;
     (void)name;
+    // the library's logs follow the app's level, as the reference's spdlog level does
+    spg_set_log_level(static_cast<int32_t>(g_level));
     try {
         const auto h_a = utils::HostCSR<T>::load_from_filename(o.input_a);
         const auto h_b = utils::HostCSR<T>::load_from_filename(o.input_b);

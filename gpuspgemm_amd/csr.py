@@ -161,6 +161,15 @@ class DeviceCSR:
                     _lib.check(lib.spg_free(p))
         self.raw = None
 
+    def free_async(self, stream: int) -> None:
+        """Release a library-owned matrix once `stream` reaches this point."""
+        if self.raw is None:
+            return
+        if not self._library_owned:
+            raise ValueError("only matrices returned by a multiply are released on a stream")
+        _lib.check(_lib.load().spg_csr_free_async(ctypes.byref(self.raw), ctypes.c_void_p(stream)))
+        self.raw = None
+
     def __del__(self):
         try:
             self.free()

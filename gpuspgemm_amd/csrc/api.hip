@@ -2,18 +2,21 @@
 //
 // proposal_inner (src/libproposal/include/proposal/proposal.cuh:26-80)
 // creates eight streams, allocates its scratch and synchronises with the
-// host at least ten times per multiply.  Here a per-device context keeps a
-// lane (four streams, a scratch arena that only grows, pinned host memory
-// for the read-backs) and the bin tables; a multiply is
+// host at least ten times per multiply.  Here a lane (a main stream, two
+// aux streams, a scratch arena that only grows, pinned host words for the
+// read-backs) is kept per calling thread or caller stream, outputs come
+// from a device block cache, and a multiply is
 //
-//   nip (with the symbolic binning), sym_multi_small || sym_multi ||
-//   sym_max (three streams), scan_count, scan_apply
-//   -> one kernel copies the counters to pinned host memory and signals;
-//      the host polls the signal (one synchronisation)
-//   -> allocate C.col / C.val of the exact size
-//   -> one symnum launch per non-empty fused bin over the four streams,
-//      then a kernel that copies the flags back, zeroes the counters and
-//      signals the host
+//   nip (with the symbolic binning)
+//   -> sym_multi_small on the main stream || sym_max, sym_multi on two aux
+//      streams (forked and joined with events)
+//   -> scan_count, scan_apply
+//   -> one kernel copies the counters to pinned host memory, zeroes them
+//      and signals; the host polls the signal (one synchronisation)
+//   -> C.col / C.val of the exact size from the block cache
+//   -> one symnum launch per non-empty fused bin over the three streams,
+//      with no stream waits (the host has seen the scan complete), each
+//      stream ending with a kernel that signals the host.
 //
 // Rows that overflow the largest LDS table add a second round trip
 // (global tables, then the scan again), as the reference's fail bin does
@@ -27,9 +30,12 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <functional>
+#include <map>
 #include <memory>
 #include <thread>
 #include <unordered_map>
+#include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -68,6 +74,49 @@ spg_status hip_fail(hipError_t e, const char* what, const char* file, int line) 
     } while (0)
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ---------------------------------------------------------------- logging
+//
+// The reference logs the bin tables, max nnz, nnz(C) and the fail-bin size
+// at debug level through spdlog (binning.cuh:181-187, 232-233,
+// sym1.cuh:430).  Levels here follow spdlog's order; the default is off.
+// The level comes from spg_set_log_level or, before the first call, from
+// the SPG_LOG_LEVEL environment variable (a name or a number).
+enum LogLevel { kTrace = 0, kDebug = 1, kInfo = 2, kWarn = 3, kError = 4, kCritical = 5, kOff = 6 };
+std::atomic<int> g_log_level{-1};
+
+int log_level() {
+    int lv = g_log_level.load(std::memory_order_relaxed);
+    if (lv >= 0) return lv;
+    lv = kOff;
+    if (const char* env = std::getenv("SPG_LOG_LEVEL")) {
+        static const char* names[] = {"trace", "debug", "info", "warn", "error", "critical", "off"};
+        for (int i = 0; i < 7; ++i)
+            if (std::strcmp(env, names[i]) == 0) lv = i;
+        if (env[0] >= '0' && env[0] <= '6' && env[1] == 0) lv = env[0] - '0';
+    }
+    int expected = -1;
+    g_log_level.compare_exchange_strong(expected, lv);
+    return g_log_level.load();
+}
+
+bool log_on(int lv) { return log_level() <= lv; }
+
+void log_msg(int lv, const char* fmt, ...) {
+    if (!log_on(lv)) return;
+    static std::mutex mu;
+    char buf[2048];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    std::lock_guard<std::mutex> g(mu);
+    std::fprintf(stderr, "[gpuspgemm %s] %s\n", lv <= kTrace ? "trace" : lv == kDebug ? "debug" : "info", buf);
+}
 
 struct ProfRec {
     std::string name;
@@ -134,24 +183,167 @@ struct GraphCache {
     }
 };
 
-// The device resources of one multiply in flight: a main stream plus aux
-// streams that the fused-pass launches are spread over, events, a scratch
-// arena that only grows, and pinned host memory for the read-backs.  The
-// context owns one lane for the plain calls; a server owns lanes of its own
-// so that the symbolic phase of one batch overlaps the fused pass of the
-// batch before it.
+// ---------------------------------------------------------------- block cache
+//
+// Device memory of the outputs (C.rpt, C.col, C.val) and of the per-call
+// global-table pools.  The stream-ordered pool (hipMallocAsync) took 3 us
+// on average and up to 80 us of host time per allocation on the multiply's
+// critical path, between the read-back of nnz(C) and the first fused
+// launch; here a released block goes back to a cache and is handed out
+// again once the GPU work queued before its release has completed (an
+// event recorded at the release, queried at reuse).  Blocks are rounded up
+// to eight size classes per power of two, and a request takes a ready block
+// of at most twice its size.  When hipMalloc fails the cache waits for the
+// device, frees its idle blocks and tries again.
+struct BlockCache {
+    struct Fence {
+        hipEvent_t ev = nullptr;
+    };
+    struct Idle {
+        void* p;
+        std::shared_ptr<Fence> fence;  // null: no GPU work can still use the block
+    };
+    std::mutex mu;
+    std::multimap<size_t, Idle> idle;              // by size
+    std::unordered_map<void*, size_t> live;        // handed-out block -> size
+    std::vector<hipEvent_t> spare;                 // events of fences no block refers to any more
+    size_t idle_bytes = 0, live_bytes = 0;
+
+    static size_t round_up(size_t b) {
+        size_t p2 = 4096;
+        while (p2 < b) p2 <<= 1;
+        const size_t g = std::max<size_t>(4096, p2 / 8);
+        return (b + g - 1) / g * g;
+    }
+
+    void drop_fence(std::shared_ptr<Fence>& f) {  // mu held
+        if (f && f.use_count() == 1 && f->ev) spare.push_back(f->ev);
+        f.reset();
+    }
+
+    spg_status alloc(void** out, size_t bytes) {
+        *out = nullptr;
+        if (bytes == 0) return SPG_OK;
+        const size_t want = round_up(bytes);
+        std::lock_guard<std::mutex> g(mu);
+        for (auto it = idle.lower_bound(want); it != idle.end() && it->first <= 2 * want; ++it) {
+            Idle& b = it->second;
+            if (b.fence && b.fence->ev) {
+                const hipError_t q = hipEventQuery(b.fence->ev);
+                if (q == hipErrorNotReady) continue;
+                if (q != hipSuccess) continue;  // leave it; a later call finds out
+            }
+            *out = b.p;
+            live[b.p] = it->first;
+            live_bytes += it->first;
+            idle_bytes -= it->first;
+            drop_fence(b.fence);
+            idle.erase(it);
+            return SPG_OK;
+        }
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+            (void)hipGetLastError();
+            (void)hipDeviceSynchronize();
+            trim_locked();
+            e = hipMalloc(&p, want);
+        }
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc (output block)", __FILE__, __LINE__);
+        live[p] = want;
+        live_bytes += want;
+        *out = p;
+        return SPG_OK;
+    }
+
+    // Back to the cache.  With a stream, the block is reused only after the
+    // work queued on that stream so far has completed.
+    spg_status release(void* p, hipStream_t after, bool use_stream) {
+        if (!p) return SPG_OK;
+        std::lock_guard<std::mutex> g(mu);
+        auto it = live.find(p);
+        if (it == live.end()) return fail(SPG_ERR_INVALID_ARGUMENT, "pointer was not allocated by the library");
+        const size_t bytes = it->second;
+        std::shared_ptr<Fence> fence;
+        if (use_stream) {
+            fence = std::make_shared<Fence>();
+            if (!spare.empty()) {
+                fence->ev = spare.back();
+                spare.pop_back();
+            } else {
+                SPG_HIP(hipEventCreateWithFlags(&fence->ev, hipEventDisableTiming));
+            }
+            SPG_HIP(hipEventRecord(fence->ev, after));
+        }
+        live.erase(it);
+        live_bytes -= bytes;
+        idle.emplace(bytes, Idle{p, std::move(fence)});
+        idle_bytes += bytes;
+        return SPG_OK;
+    }
+
+    // Several blocks released behind one event.
+    spg_status release_all(void* const* ps, int n, hipStream_t after) {
+        std::lock_guard<std::mutex> g(mu);
+        std::shared_ptr<Fence> fence = std::make_shared<Fence>();
+        if (!spare.empty()) {
+            fence->ev = spare.back();
+            spare.pop_back();
+        } else {
+            SPG_HIP(hipEventCreateWithFlags(&fence->ev, hipEventDisableTiming));
+        }
+        SPG_HIP(hipEventRecord(fence->ev, after));
+        for (int i = 0; i < n; ++i) {
+            if (!ps[i]) continue;
+            auto it = live.find(ps[i]);
+            if (it == live.end()) return fail(SPG_ERR_INVALID_ARGUMENT, "pointer was not allocated by the library");
+            live_bytes -= it->second;
+            idle_bytes += it->second;
+            idle.emplace(it->second, Idle{ps[i], fence});
+            live.erase(it);
+        }
+        return SPG_OK;
+    }
+
+    void trim_locked() {  // device idle
+        for (auto& kv : idle) {
+            drop_fence(kv.second.fence);
+            (void)hipFree(kv.second.p);
+        }
+        idle.clear();
+        idle_bytes = 0;
+        for (hipEvent_t e : spare) (void)hipEventDestroy(e);
+        spare.clear();
+    }
+};
+
+// ---------------------------------------------------------------- lanes
+
+// Pinned host words a stream's last kernel fills: the fused pass's flags
+// word and the completion flag, on a line of their own.
+struct alignas(64) DoneWord {
+    int flag;
+    int flags;
+};
+
+// The device resources of one multiply in flight: aux streams that the
+// symbolic phase forks onto and the fused-pass launches are spread over,
+// events for the forks and joins, a scratch arena that only grows, and
+// pinned host memory for the read-backs.  The main stream is the caller's
+// (a thread's default stream for the synchronous calls) or the lane's own.
 struct Lane {
-    static constexpr int kStreams = 4;
-    hipStream_t s = nullptr;
-    hipStream_t aux[kStreams - 1] = {};
-    hipEvent_t ev_ready = nullptr;
-    hipEvent_t ev_done[kStreams - 1] = {};
-    hipEvent_t ev_host = nullptr;  // error check while the host polls
-    int* h_flag = nullptr;          // pinned: cleared by the host, set to 1 by k_signal
+    static constexpr int kAux = 2;
+    static constexpr int kStreams = kAux + 1;
+    std::mutex mu;  // one call at a time
+    hipStream_t own = nullptr;
+    hipStream_t aux[kAux] = {};
+    hipEvent_t ev_ready = nullptr;            // fork point on the main stream
+    hipEvent_t ev_done[kAux] = {};            // join points on the aux streams
+    DoneWord* h_done = nullptr;               // pinned, kStreams words
     char* scratch = nullptr;
     size_t scratch_cap = 0;
     bool meta_clean = false;  // the counters at the start of scratch are zero
-    char* gpool = nullptr;  // global-bin tables of the fused pass under graph replay
+    char* gpool = nullptr;    // global-bin tables of the fused pass under graph replay
     size_t gpool_cap = 0;
     DevMeta* h_meta = nullptr;
     char* h_stage = nullptr;
@@ -159,21 +351,100 @@ struct Lane {
     GraphCache graphs[2];  // symbolic phase, fused pass
 };
 
+spg_status lane_create(Lane& L) {
+    SPG_HIP(hipStreamCreateWithFlags(&L.own, hipStreamNonBlocking));
+    for (int i = 0; i < Lane::kAux; ++i) {
+        SPG_HIP(hipStreamCreateWithFlags(&L.aux[i], hipStreamNonBlocking));
+        SPG_HIP(hipEventCreateWithFlags(&L.ev_done[i], hipEventDisableTiming));
+    }
+    SPG_HIP(hipEventCreateWithFlags(&L.ev_ready, hipEventDisableTiming));
+    SPG_HIP(hipHostMalloc(reinterpret_cast<void**>(&L.h_done), sizeof(DoneWord) * Lane::kStreams,
+                          hipHostMallocDefault));
+    std::memset(L.h_done, 0, sizeof(DoneWord) * Lane::kStreams);
+    SPG_HIP(hipHostMalloc(reinterpret_cast<void**>(&L.h_meta), sizeof(DevMeta), hipHostMallocDefault));
+    return SPG_OK;
+}
+
+void lane_destroy(Lane& L) {
+    if (L.own) (void)hipStreamSynchronize(L.own);
+    for (int i = 0; i < Lane::kAux; ++i) {
+        if (L.aux[i]) (void)hipStreamSynchronize(L.aux[i]);
+        if (L.aux[i]) (void)hipStreamDestroy(L.aux[i]);
+        if (L.ev_done[i]) (void)hipEventDestroy(L.ev_done[i]);
+    }
+    if (L.ev_ready) (void)hipEventDestroy(L.ev_ready);
+    L.graphs[0].clear();
+    L.graphs[1].clear();
+    if (L.h_done) (void)hipHostFree(L.h_done);
+    if (L.scratch) (void)hipFree(L.scratch);
+    if (L.gpool) (void)hipFree(L.gpool);
+    if (L.h_meta) (void)hipHostFree(L.h_meta);
+    if (L.h_stage) (void)hipHostFree(L.h_stage);
+    if (L.own) (void)hipStreamDestroy(L.own);
+    L.own = nullptr;
+    for (auto& a : L.aux) a = nullptr;
+    L.h_done = nullptr;
+    L.scratch = nullptr;
+    L.scratch_cap = 0;
+    L.gpool = nullptr;
+    L.gpool_cap = 0;
+    L.h_meta = nullptr;
+    L.h_stage = nullptr;
+    L.h_stage_cap = 0;
+}
+
+// Completion of the work queued on some of a lane's streams.  The host
+// clears a word of pinned host memory per stream (lane_arm), a kernel
+// queued last on that stream stores 1 into it (k_readback, eagerly or as a
+// graph's last node) and the host polls the words (lane_poll): the
+// runtime's own completion of an event or stream was measured at ~55 us
+// after the GPU finished.  The streams are queried now and then so that a
+// failed stream ends the wait with its error.  Every signal is polled for
+// before the lane is armed again, so no earlier store can land after the
+// clear.
+void lane_arm(Lane& L, int q) {
+    L.h_done[q].flags = 0;
+    __atomic_store_n(&L.h_done[q].flag, 0, __ATOMIC_RELEASE);
+}
+
+hipError_t lane_poll(Lane& L, int mask, const hipStream_t* streams) {
+    for (unsigned spin = 1;; ++spin) {
+        bool all = true;
+        for (int q = 0; q < Lane::kStreams; ++q)
+            if ((mask >> q & 1) && __atomic_load_n(&L.h_done[q].flag, __ATOMIC_ACQUIRE) != 1) all = false;
+        if (all) return hipSuccess;
+        if ((spin & 1023) == 0) {
+            // a stream that has completed without storing its word failed;
+            // the others keep being waited for
+            for (int q = 0; q < Lane::kStreams; ++q) {
+                if (!(mask >> q & 1) || __atomic_load_n(&L.h_done[q].flag, __ATOMIC_ACQUIRE) == 1) continue;
+                const hipError_t e = hipStreamQuery(streams[q]);
+                if (e == hipSuccess && __atomic_load_n(&L.h_done[q].flag, __ATOMIC_ACQUIRE) != 1) return hipErrorUnknown;
+                if (e != hipSuccess && e != hipErrorNotReady) return e;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- context
+
 struct Context {
-    bool ready = false;
+    std::atomic<bool> ready{false};
     int device = -1;
-    hipStream_t stream = nullptr;  // the context lane's main stream
     int cus = 0;
     int lds = 0;
     int clock_khz = 0;
     size_t global_mem = 0;
     std::string arch;
-    bool async_alloc = true;
+    std::mutex mu;  // tables, the lane map, profiling records
     BinTable sym{};
     BinTable num{};
     int occ_sym_multi[2] = {1, 1}, occ_sym_max = 1;
     int occ_num[2][kNumLdsVariants] = {};
-    Lane lane;
+    Lane lane;  // graph replay and profiling run here, one call at a time
+    std::unordered_map<uint64_t, std::unique_ptr<Lane>> lanes;  // per caller stream
+    std::vector<std::unique_ptr<Lane>> free_lanes;              // lanes of threads that ended
+    BlockCache cache;
     bool graphs = false;  // replay repeated phases from captured graphs
     std::atomic<int64_t> n_captured{0}, n_replayed{0};
     bool profiling = false;
@@ -182,9 +453,15 @@ struct Context {
     size_t events_used = 0;
 };
 
+// Initialisation, graph capture and profiling, and the synchronous helpers
+// (which must not run beside a capture) take this lock; plain multiplies
+// from different threads or streams do not.
 std::recursive_mutex g_mutex;
 constexpr int kMaxDevices = 64;
-Context g_ctx[kMaxDevices];
+// Contexts live until the process ends and are never destroyed: their
+// lanes and cached blocks stay reachable, and no HIP call runs after the
+// runtime has shut down.
+std::atomic<Context*> g_ctx[kMaxDevices];
 
 // ---------------------------------------------------------------- bin tables
 
@@ -245,79 +522,31 @@ spg_status check_num_table(const BinTable& t) {
     return SPG_OK;
 }
 
-// ---------------------------------------------------------------- context
-
-spg_status lane_create(Lane& L) {
-    SPG_HIP(hipStreamCreateWithFlags(&L.s, hipStreamNonBlocking));
-    for (int i = 0; i < Lane::kStreams - 1; ++i) {
-        SPG_HIP(hipStreamCreateWithFlags(&L.aux[i], hipStreamNonBlocking));
-        SPG_HIP(hipEventCreateWithFlags(&L.ev_done[i], hipEventDisableTiming));
-    }
-    SPG_HIP(hipEventCreateWithFlags(&L.ev_ready, hipEventDisableTiming));
-    SPG_HIP(hipEventCreateWithFlags(&L.ev_host, hipEventDisableTiming));
-    SPG_HIP(hipHostMalloc(reinterpret_cast<void**>(&L.h_flag), sizeof(int), hipHostMallocDefault));
-    *L.h_flag = 0;
-    SPG_HIP(hipHostMalloc(reinterpret_cast<void**>(&L.h_meta), sizeof(DevMeta), hipHostMallocDefault));
-    return SPG_OK;
-}
-
-void lane_destroy(Lane& L) {
-    if (L.s) (void)hipStreamSynchronize(L.s);
-    for (int i = 0; i < Lane::kStreams - 1; ++i) {
-        if (L.aux[i]) (void)hipStreamSynchronize(L.aux[i]);
-        if (L.aux[i]) (void)hipStreamDestroy(L.aux[i]);
-        if (L.ev_done[i]) (void)hipEventDestroy(L.ev_done[i]);
-    }
-    if (L.ev_ready) (void)hipEventDestroy(L.ev_ready);
-    if (L.ev_host) (void)hipEventDestroy(L.ev_host);
-    L.graphs[0].clear();
-    L.graphs[1].clear();
-    if (L.h_flag) (void)hipHostFree(L.h_flag);
-    if (L.scratch) (void)hipFree(L.scratch);
-    if (L.gpool) (void)hipFree(L.gpool);
-    if (L.h_meta) (void)hipHostFree(L.h_meta);
-    if (L.h_stage) (void)hipHostFree(L.h_stage);
-    if (L.s) (void)hipStreamDestroy(L.s);
-    L = Lane{};
-}
-
-// Completion of the work queued on a lane's main stream.  The host clears a
-// word of pinned host memory (lane_arm), a kernel queued last stores 1 into
-// it (launch_signal, eagerly or as a graph's last node) and the host polls
-// the word (lane_poll): the runtime's own completion of an event or stream
-// was measured at ~55 us after the GPU finished, at each of the multiply's
-// two read-backs.  The event is queried now and then so that a failed stream
-// ends the wait with its error.  Every signal is polled for before the lane
-// is armed again, so no earlier store can land after the clear.
-void lane_arm(Lane& L) { __atomic_store_n(L.h_flag, 0, __ATOMIC_RELEASE); }
-
-hipError_t lane_poll(Lane& L) {
-    hipError_t e = hipEventRecord(L.ev_host, L.s);
-    if (e != hipSuccess) return e;
-    for (unsigned spin = 1;; ++spin) {
-        if (__atomic_load_n(L.h_flag, __ATOMIC_ACQUIRE) == 1) return hipSuccess;
-        if ((spin & 1023) == 0) {
-            e = hipEventQuery(L.ev_host);
-            if (e == hipSuccess) return __atomic_load_n(L.h_flag, __ATOMIC_ACQUIRE) == 1 ? hipSuccess : hipErrorUnknown;
-            if (e != hipErrorNotReady) return e;
-        }
-    }
-}
-
-hipError_t lane_wait(Lane& L) {
-    lane_arm(L);
-    launch_signal(L.h_flag, L.s);
-    const hipError_t e = hipGetLastError();
-    return e == hipSuccess ? lane_poll(L) : e;
+std::string table_text(const BinTable& t) {
+    std::ostringstream out;
+    for (int b = 0; b < t.n; ++b)
+        out << (b ? ", " : "") << "v" << t.variant[b] << " nip<=" << t.nip_cap[b] << " nnz<=" << t.nnz_cap[b];
+    return out.str();
 }
 
 spg_status ensure_context(Context*& out) {
     int dev = 0;
     SPG_HIP(hipGetDevice(&dev));
     if (dev < 0 || dev >= kMaxDevices) return fail(SPG_ERR_UNSUPPORTED_DEVICE, "device index out of range");
-    Context& c = g_ctx[dev];
-    out = &c;
-    if (c.ready) return SPG_OK;
+    Context* cp = g_ctx[dev].load(std::memory_order_acquire);
+    if (cp && cp->ready.load(std::memory_order_acquire)) {
+        out = cp;
+        return SPG_OK;
+    }
+    std::lock_guard<std::recursive_mutex> lock(g_mutex);
+    cp = g_ctx[dev].load(std::memory_order_acquire);
+    if (!cp) {
+        cp = new Context();
+        g_ctx[dev].store(cp, std::memory_order_release);
+    }
+    Context& c = *cp;
+    out = cp;
+    if (c.ready.load(std::memory_order_acquire)) return SPG_OK;
     hipDeviceProp_t prop;
     SPG_HIP(hipGetDeviceProperties(&prop, dev));
     c.arch = prop.gcnArchName;
@@ -331,16 +560,7 @@ spg_status ensure_context(Context*& out) {
     if (c.lds < kNumBigLds + 1024 ||
         c.lds < kSymVariants[kSymMaxVariant].t * 4 + 8 * kSymVariants[kSymMaxVariant].ec + 1024)
         return fail(SPG_ERR_UNSUPPORTED_DEVICE, "device offers less LDS per workgroup than the kernels need");
-    // Keep freed blocks in the stream-ordered pool: C is allocated per call.
-    hipMemPool_t pool;
-    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
-        uint64_t threshold = UINT64_MAX;
-        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
-    } else {
-        c.async_alloc = false;
-    }
     SPG_TRY(lane_create(c.lane));
-    c.stream = c.lane.s;
     default_tables(c.sym, c.num);
     SPG_TRY(check_sym_table(c.sym));  // a stale tuned_tables.inc fails here, loudly
     SPG_TRY(check_num_table(c.num));
@@ -351,33 +571,69 @@ spg_status ensure_context(Context*& out) {
         c.occ_num[0][v] = occupancy_symnum_bin<float>(v);
         c.occ_num[1][v] = occupancy_symnum_bin<double>(v);
     }
-    c.ready = true;
+    log_msg(kDebug, "context on device %d (%s, %d CUs); symbolic bins: %s; fused bins: %s", dev, c.arch.c_str(),
+            c.cus, table_text(c.sym).c_str(), table_text(c.num).c_str());
+    c.ready.store(true, std::memory_order_release);
     return SPG_OK;
 }
 
-spg_status dev_alloc(Context& c, void** p, size_t bytes, hipStream_t s) {
-    *p = nullptr;
-    if (bytes == 0) return SPG_OK;
-    if (c.async_alloc) {
-        const hipError_t e = hipMallocAsync(p, bytes, s);
-        if (e == hipSuccess) return SPG_OK;
-        if (e != hipErrorNotSupported) return hip_fail(e, "hipMallocAsync", __FILE__, __LINE__);
-        c.async_alloc = false;
+// A thread's lane for its synchronous calls goes back to the context's
+// spare list when the thread ends, so that threads that come and go reuse
+// lanes instead of adding new ones.
+struct ThreadLanes {
+    std::vector<std::pair<Context*, uint64_t>> held;
+    ~ThreadLanes() {
+        for (auto& [c, key] : held) {
+            std::lock_guard<std::mutex> g(c->mu);
+            auto it = c->lanes.find(key);
+            if (it == c->lanes.end()) continue;
+            c->free_lanes.push_back(std::move(it->second));
+            c->lanes.erase(it);
+        }
     }
-    SPG_HIP(hipMalloc(p, bytes));
+};
+
+uint64_t thread_key() {
+    static std::atomic<uint64_t> next{1};
+    thread_local const uint64_t id = next.fetch_add(1);
+    return (id << 1) | 1;  // odd: never a stream handle
+}
+
+// The lane of a caller stream, or of the calling thread when `stream` is
+// null (its default stream).
+spg_status lane_for(Context& c, hipStream_t stream, Lane*& out) {
+    const uint64_t key = stream ? reinterpret_cast<uint64_t>(stream) : thread_key();
+    std::lock_guard<std::mutex> g(c.mu);
+    auto it = c.lanes.find(key);
+    if (it != c.lanes.end()) {
+        out = it->second.get();
+        return SPG_OK;
+    }
+    std::unique_ptr<Lane> L;
+    if (!c.free_lanes.empty()) {
+        L = std::move(c.free_lanes.back());
+        c.free_lanes.pop_back();
+    } else {
+        L = std::make_unique<Lane>();
+        const spg_status st = lane_create(*L);
+        if (st != SPG_OK) {
+            lane_destroy(*L);
+            return st;
+        }
+    }
+    out = L.get();
+    c.lanes.emplace(key, std::move(L));
+    if (!stream) {
+        thread_local ThreadLanes mine;
+        mine.held.emplace_back(&c, key);
+    }
     return SPG_OK;
 }
 
-spg_status dev_release(Context& c, void* p, hipStream_t s) {
-    if (!p) return SPG_OK;
-    if (c.async_alloc) SPG_HIP(hipFreeAsync(p, s));
-    else SPG_HIP(hipFree(p));
-    return SPG_OK;
-}
-
-spg_status ensure_scratch(Lane& L, size_t bytes) {
+spg_status ensure_scratch(Lane& L, size_t bytes, hipStream_t s) {
     if (bytes <= L.scratch_cap) return SPG_OK;
-    SPG_HIP(hipStreamSynchronize(L.s));
+    SPG_HIP(hipStreamSynchronize(s));
+    for (hipStream_t a : L.aux) SPG_HIP(hipStreamSynchronize(a));
     if (L.scratch) SPG_HIP(hipFree(L.scratch));
     L.scratch = nullptr;
     const size_t cap = std::max(bytes, L.scratch_cap * 3 / 2);
@@ -388,9 +644,10 @@ spg_status ensure_scratch(Lane& L, size_t bytes) {
     return SPG_OK;
 }
 
-spg_status ensure_gpool(Lane& L, size_t bytes) {
+spg_status ensure_gpool(Lane& L, size_t bytes, hipStream_t s) {
     if (bytes <= L.gpool_cap) return SPG_OK;
-    SPG_HIP(hipStreamSynchronize(L.s));
+    SPG_HIP(hipStreamSynchronize(s));
+    for (hipStream_t a : L.aux) SPG_HIP(hipStreamSynchronize(a));
     if (L.gpool) SPG_HIP(hipFree(L.gpool));
     L.gpool = nullptr;
     const size_t cap = std::max(bytes, L.gpool_cap * 3 / 2);
@@ -400,9 +657,9 @@ spg_status ensure_gpool(Lane& L, size_t bytes) {
     return SPG_OK;
 }
 
-spg_status ensure_stage(Lane& L, size_t bytes) {
+spg_status ensure_stage(Lane& L, size_t bytes, hipStream_t s) {
     if (bytes <= L.h_stage_cap) return SPG_OK;
-    SPG_HIP(hipStreamSynchronize(L.s));
+    SPG_HIP(hipStreamSynchronize(s));
     if (L.h_stage) SPG_HIP(hipHostFree(L.h_stage));
     L.h_stage = nullptr;
     L.h_stage_cap = 0;
@@ -428,6 +685,8 @@ struct Prof {
     }
     hipStream_t st = nullptr;
     void begin(const std::string& name, int64_t rows, int grid, hipStream_t where) {
+        if (log_on(kTrace))
+            log_msg(kTrace, "launch %s: %lld rows, grid %d", name.c_str(), static_cast<long long>(rows), grid);
         if (!on) return;
         st = where;
         ProfRec r{name, rows, grid, next_event(), next_event(), 0.0};
@@ -468,13 +727,22 @@ spg_status check_operands(const spg_csr& A, const spg_csr& B) {
     return SPG_OK;
 }
 
+// How a call's work is ordered and waited for.
+struct CallMode {
+    hipStream_t main = nullptr;  // the stream the multiply is ordered on
+    bool own_stream = false;     // `main` is the lane's own stream
+    bool events = false;         // fused pass forked and joined with events, one read-back at the end (graph capture)
+    bool async = false;          // return once the fused pass is queued; C is ready in `main`'s order
+                                 // (single products only: no staged descriptors outlive the call)
+};
+
 // One batch of products on one lane, in two phases:
 //   symbolic()        NIP, binning, exact counts, scan (row pointers) and
 //                     the fused-pass bins; ends with one read-back, after
 //                     which nnz(C_i) of every product is known on the host
-//   launch_numeric()  allocates C.col / C.val of the exact sizes and queues
-//                     the fused pass on the lane's streams (no wait)
-//   finish()          waits for the lane and checks the consistency flag.
+//   launch_numeric()  takes C.col / C.val of the exact sizes from the block
+//                     cache and queues the fused pass (no wait)
+//   finish()          waits for the streams and checks the consistency flag.
 // A plain call runs the three back to back; the server admits a batch by
 // output_bytes() between the first two.
 struct JobBase {
@@ -491,6 +759,8 @@ struct Job : JobBase {
     Context& c;
     Lane& L;
     Prof prof;
+    CallMode mode;
+    hipStream_t s = nullptr;
     int count = 0;
     spg_csr* C = nullptr;
     BinTable sym{}, num{};
@@ -517,30 +787,42 @@ struct Job : JobBase {
     // over-aligned; a Job is deleted through its base)
     struct {
         int32_t flags = 0;
+        int32_t sym_count[kMaxBins] = {};
         int32_t num_count[kMaxBins] = {};
+        int32_t fail_count = 0;
+        int32_t max_nip = 0, max_nnz = 0;
+        unsigned long long total_nip = 0;
         unsigned long long gnum_pool = 0;
         long long nnz_total = 0;
     } hm;
     std::vector<long long> nnz;
-    void* num_pool = nullptr;   // allocated per call (eager) or the lane's pool (graph replay)
+    void* num_pool = nullptr;   // from the block cache (eager) or the lane's pool (graph replay)
     bool graphs = false;        // phases may be replayed from the lane's graph caches
-    bool signal_queued = false; // the fused pass already queued its completion signal
+    bool signal_queued = false; // the fused pass already queued its completion signals
+    int done_mask = 0;          // streams whose completion words the fused pass fills
+    double t_start = 0, t_sym = 0;
 
     // Only the context's own lane captures graphs: its calls run under the
     // library mutex, which the library's synchronous helpers (spg_memcpy,
     // spg_malloc, spg_free, spg_device_sync) take too.  HIP rejects a
     // synchronous call on the legacy stream from any thread while some
-    // stream of the device is capturing, so a server's worker threads,
-    // which run beside other threads, never capture.
-    Job(Context& ctx, Lane& lane, const BinTable& sym_table, const BinTable& num_table, bool profile)
-        : c(ctx), L(lane), prof(ctx, profile), sym(sym_table), num(num_table),
-          graphs(&lane == &ctx.lane && ctx.graphs && !profile) {}
+    // stream of the device is capturing, so lanes of other threads and a
+    // server's worker threads never capture.
+    Job(Context& ctx, Lane& lane, const BinTable& sym_table, const BinTable& num_table, bool profile, CallMode m)
+        : c(ctx), L(lane), prof(ctx, profile), mode(m), s(m.main), sym(sym_table), num(num_table),
+          graphs(&lane == &ctx.lane && ctx.graphs && !profile) {
+        if (graphs) mode.events = true;
+    }
+
+    hipStream_t stream_of(int q) const { return q == 0 ? s : L.aux[q - 1]; }
 
     void abort() override {
-        for (void* p : owned) (void)dev_release(c, p, L.s);
+        // kernels queued before the failure may still use the blocks
+        if (!owned.empty()) (void)c.cache.release_all(owned.data(), static_cast<int>(owned.size()), s);
         owned.clear();
-        if (num_pool && !graphs) (void)dev_release(c, num_pool, L.s);
+        if (num_pool && !graphs) (void)c.cache.release(num_pool, s, true);
         num_pool = nullptr;
+        L.meta_clean = false;
         for (int i = 0; i < count; ++i) C[i] = spg_csr{0, 0, 0, nullptr, nullptr, nullptr};
     }
 
@@ -555,17 +837,19 @@ struct Job : JobBase {
         return b;
     }
 
+    // The stream `to` waits for the work queued on `from` so far.  (Waits
+    // on a value in memory, hipStreamWaitValue64, are served by a spinning
+    // blit kernel on this runtime; measured, it slowed the concurrent
+    // stencil count pass from 1.36 to 2.2 ms, so joins use events.)
+    hipError_t join(hipStream_t from, hipEvent_t ev, hipStream_t to) {
+        hipError_t e = hipEventRecord(ev, from);
+        return e == hipSuccess ? hipStreamWaitEvent(to, ev, 0) : e;
+    }
+
     spg_status front() {
-        const hipStream_t s = L.s;
-        if (batch) {
-            SPG_HIP(hipMemcpyAsync(d_probs, L.h_stage + stage_probs, sizeof(Prob<V>) * count, hipMemcpyHostToDevice,
-                                   s));
-            SPG_HIP(hipMemcpyAsync(d_tiles, L.h_stage + stage_tiles, sizeof(Tile) * n_tiles, hipMemcpyHostToDevice,
-                                   s));
-        }
-        // The counters are cleared by the last kernel of a multiply that
-        // completed; after anything else (first use, grown scratch, an error
-        // or an abandoned job) they are cleared here.
+        // The counters are zeroed by the symbolic phase's read-back; after
+        // anything else (first use, grown scratch, an error or an abandoned
+        // job) they are cleared here.
         if (graphs || !L.meta_clean) {
             prof.begin("clear", 0, 1, s);
             launch_clear(meta, 0, s);
@@ -576,8 +860,12 @@ struct Job : JobBase {
         launch_nip<V>(ps, sym, meta, items1, static_cast<int>(total_rows), s);
         prof.end();
         // The largest-table bin runs on an aux stream beside the multi-bin
-        // launch (it is queued first: its workgroups without a row exit at
-        // once and leave the CUs to the multi-bin grid).
+        // launches (queued first: its workgroups take the CUs they need
+        // before the multi-bin grids fill them, and those without a row
+        // exit at once).  With both multi-bin parts in use the larger-table
+        // part runs on the second aux stream: its workgroups take the CUs
+        // that the small-table grid leaves as its last units drain.  The
+        // three launches count disjoint rows.
         SPG_HIP(hipEventRecord(L.ev_ready, s));
         SPG_HIP(hipStreamWaitEvent(L.aux[0], L.ev_ready, 0));
         {
@@ -590,9 +878,6 @@ struct Job : JobBase {
         const int n_multi = sym.n - 1;
         bool part_used[2] = {false, false};
         for (int b = 0; b < n_multi; ++b) part_used[sym.variant[b] < kSymSmallVariants ? 0 : 1] = true;
-        // With both parts in use the larger-table part runs on a second aux
-        // stream: its workgroups take the CUs that the small-table grid
-        // leaves as its last units drain.  The parts count disjoint rows.
         const bool split = part_used[0] && part_used[1];
         if (split) SPG_HIP(hipStreamWaitEvent(L.aux[1], L.ev_ready, 0));
         for (int part = 0; part < 2; ++part) {
@@ -604,19 +889,15 @@ struct Job : JobBase {
             launch_sym_multi<V>(part, ps, items1, static_cast<int>(total_rows), sym, n_multi, meta, grid, where);
             prof.end();
         }
-        SPG_HIP(hipEventRecord(L.ev_done[0], L.aux[0]));
-        SPG_HIP(hipStreamWaitEvent(s, L.ev_done[0], 0));
-        if (split) {
-            SPG_HIP(hipEventRecord(L.ev_done[1], L.aux[1]));
-            SPG_HIP(hipStreamWaitEvent(s, L.ev_done[1], 0));
-        }
+        SPG_HIP(join(L.aux[0], L.ev_done[0], s));
+        if (split) SPG_HIP(join(L.aux[1], L.ev_done[1], s));
         return SPG_OK;
     }
 
-    // scan stage and the read-back of the counters, ending with the
-    // completion signal (the caller armed the lane and polls it)
+    // scan stage and the read-back of the counters (zeroing them for the
+    // next multiply), ending with the completion signal of the main stream
+    // (the caller armed it and polls it)
     spg_status scan(int after_fail) {
-        const hipStream_t s = L.s;
         prof.begin("scan_count", total_rows, n_tiles, s);
         launch_scan_count<V>(ps, num, meta, tile_sums, after_fail, s);
         prof.end();
@@ -626,7 +907,8 @@ struct Job : JobBase {
         // counters (and nnz of every product of a batch) to pinned host memory, then the signal
         launch_readback(reinterpret_cast<const int*>(meta), reinterpret_cast<int*>(L.h_meta),
                         static_cast<int>(sizeof(DevMeta) / sizeof(int)), inst_nnz,
-                        reinterpret_cast<long long*>(L.h_stage + stage_inst), batch ? count : 0, nullptr, L.h_flag, s);
+                        reinterpret_cast<long long*>(L.h_stage + stage_inst), batch ? count : 0, meta,
+                        &L.h_done[0].flag, s);
         SPG_HIP(hipGetLastError());
         return SPG_OK;
     }
@@ -655,7 +937,6 @@ struct Job : JobBase {
     // replayed from then on: one launch in place of a dozen.
     template <typename F>
     spg_status queue_phase(int phase, F&& record) {
-        lane_arm(L);
         if (!graphs) return record();
         std::vector<int64_t> sig = signature(phase);
         GraphCache& gc = L.graphs[phase];
@@ -664,10 +945,10 @@ struct Job : JobBase {
             ++c.n_replayed;
         } else {
             if (!gc.seen_before(sig)) return record();
-            SPG_HIP(hipStreamBeginCapture(L.s, hipStreamCaptureModeRelaxed));
+            SPG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
             spg_status st = record();
             hipGraph_t g = nullptr;
-            const hipError_t e = hipStreamEndCapture(L.s, &g);
+            const hipError_t e = hipStreamEndCapture(s, &g);
             if (st == SPG_OK && e != hipSuccess) st = hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
             if (st == SPG_OK) {
                 const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
@@ -678,15 +959,15 @@ struct Job : JobBase {
             gc.insert(std::move(sig), exec);
             ++c.n_captured;
         }
-        SPG_HIP(hipGraphLaunch(exec, L.s));
+        SPG_HIP(hipGraphLaunch(exec, s));
         return SPG_OK;
     }
 
     spg_status symbolic(int n, const spg_csr* A, const spg_csr* B, spg_csr* out) override {
+        t_start = now_s();
         count = n;
         C = out;
         batch = count > 1;
-        const hipStream_t s = L.s;
         for (int i = 0; i < count; ++i) {
             SPG_TRY(check_operands(A[i], B[i]));
             C[i] = spg_csr{0, A[i].m, B[i].n, nullptr, nullptr, nullptr};
@@ -694,7 +975,7 @@ struct Job : JobBase {
         // output row pointers
         for (int i = 0; i < count; ++i) {
             void* p = nullptr;
-            const spg_status st = dev_alloc(c, &p, sizeof(int32_t) * (static_cast<size_t>(A[i].m) + 1), s);
+            const spg_status st = c.cache.alloc(&p, sizeof(int32_t) * (static_cast<size_t>(A[i].m) + 1));
             if (st != SPG_OK) return failed(st);
             owned.push_back(p);
             C[i].rpt = static_cast<int32_t*>(p);
@@ -731,10 +1012,12 @@ struct Job : JobBase {
         const size_t o_fit = take(sizeof(int2) * M);
         const size_t o_foff = take(sizeof(long long) * M);
         const size_t o_goff = take(sizeof(long long) * M);
-        const size_t o_probs = take(batch ? sizeof(Prob<V>) * count : 0);
+        // descriptors: a table per stream of the fused pass, each copied on
+        // its own stream (the eager fused pass has no stream waits)
+        const size_t o_probs = take(batch ? sizeof(Prob<V>) * count * Lane::kStreams : 0);
         const size_t o_tiles = take(batch ? sizeof(Tile) * n_tiles : 0);
         {
-            const spg_status st = ensure_scratch(L, off);
+            const spg_status st = ensure_scratch(L, off, s);
             if (st != SPG_OK) return failed(st);
         }
         char* base = L.scratch;
@@ -783,67 +1066,103 @@ struct Job : JobBase {
             stage_probs = 0;
             stage_tiles = align256(sizeof(Prob<V>) * count);
             stage_inst = stage_tiles + align256(sizeof(Tile) * n_tiles);
-            const spg_status st = ensure_stage(L, stage_inst + align256(sizeof(long long) * count));
+            const spg_status st = ensure_stage(L, stage_inst + align256(sizeof(long long) * count), s);
             if (st != SPG_OK) return failed(st);
             std::memcpy(L.h_stage + stage_probs, probs.data(), sizeof(Prob<V>) * count);
             std::memcpy(L.h_stage + stage_tiles, tiles.data(), sizeof(Tile) * n_tiles);
         }
 
+        lane_arm(L, 0);
         spg_status st = queue_phase(0, [&]() -> spg_status {
+            if (batch) {
+                SPG_HIP(hipMemcpyAsync(d_probs, L.h_stage + stage_probs, sizeof(Prob<V>) * count,
+                                       hipMemcpyHostToDevice, s));
+                SPG_HIP(hipMemcpyAsync(d_tiles, L.h_stage + stage_tiles, sizeof(Tile) * n_tiles,
+                                       hipMemcpyHostToDevice, s));
+            }
             SPG_TRY(front());
             return scan(0);
         });
+        hipStream_t main_only[Lane::kStreams];
+        for (hipStream_t& m : main_only) m = s;
         if (st == SPG_OK) {
-            const hipError_t e = lane_poll(L);
+            const hipError_t e = lane_poll(L, 1, main_only);
             if (e != hipSuccess) st = hip_fail(e, "symbolic phase", __FILE__, __LINE__);
+        }
+        if (st == SPG_OK && (L.h_meta->flags.v & kFlagMismatch)) {
+            // left by the fused pass of an asynchronous multiply queued earlier on this stream
+            st = fail(SPG_ERR_INTERNAL, "the fused pass of an earlier multiply on this stream found a different "
+                                        "column count than its count pass (are B's rows free of duplicate "
+                                        "columns and indices in range?)");
         }
         if (st == SPG_OK && L.h_meta->fail_count.v > 0) {
             // rows that overflowed the largest LDS table: count them again in
             // global memory, then redo the scan stage
             const int fails = L.h_meta->fail_count.v;
+            hm.fail_count = fails;
             void* sym_pool = nullptr;
-            st = dev_alloc(c, &sym_pool, sizeof(int32_t) * L.h_meta->fail_pool.v, s);
+            st = c.cache.alloc(&sym_pool, sizeof(int32_t) * L.h_meta->fail_pool.v);
             if (st == SPG_OK) {
                 prof.begin("sym_global", fails, fails, s);
                 launch_sym_global<V>(ps, fail_items, fail_off, static_cast<int*>(sym_pool), fails, s);
                 prof.end();
                 launch_clear(meta, kMetaScanWord, s);
-                lane_arm(L);
+                lane_arm(L, 0);
                 st = scan(1);
                 if (st == SPG_OK) {
-                    const hipError_t e = lane_poll(L);
+                    const hipError_t e = lane_poll(L, 1, main_only);
                     if (e != hipSuccess) st = hip_fail(e, "symbolic phase", __FILE__, __LINE__);
                 }
             }
-            if (sym_pool) (void)dev_release(c, sym_pool, s);
+            // the main stream has completed (or failed): the pool is idle, or
+            // waits for the stream
+            if (sym_pool) (void)c.cache.release(sym_pool, s, st != SPG_OK);
         }
         if (st != SPG_OK) return failed(st);
         hm.flags = L.h_meta->flags.v;
-        for (int b = 0; b < kMaxBins; ++b) hm.num_count[b] = L.h_meta->num_count[b].v;
+        for (int b = 0; b < kMaxBins; ++b) {
+            hm.num_count[b] = L.h_meta->num_count[b].v;
+            hm.sym_count[b] = L.h_meta->sym_count[b].v;
+        }
         hm.gnum_pool = L.h_meta->gnum_pool.v;
         hm.nnz_total = L.h_meta->nnz_total.v;
+        hm.max_nip = L.h_meta->max_nip.v;
+        hm.max_nnz = L.h_meta->max_nnz.v;
+        hm.total_nip = L.h_meta->total_nip.v;
         if (hm.flags & kFlagNipOverflow)
             return failed(fail(SPG_ERR_OVERFLOW, "a row has 2^31 or more intermediate products"));
         if (hm.flags & kFlagNnzOverflow) return failed(fail(SPG_ERR_OVERFLOW, "nnz(C) does not fit int32"));
+        long long all_nnz = 0;
         for (int i = 0; i < count; ++i) {
             if (A[i].m == 0) continue;
             nnz[i] = batch ? reinterpret_cast<long long*>(L.h_stage + stage_inst)[i] : hm.nnz_total;
             if (nnz[i] >= INT_MAX) return failed(fail(SPG_ERR_OVERFLOW, "nnz(C) does not fit int32"));
+            all_nnz += nnz[i];
+        }
+        t_sym = now_s();
+        if (log_on(kDebug)) {
+            std::ostringstream sb, nb;
+            for (int b = 0; b < sym.n; ++b) sb << (b ? " " : "") << hm.sym_count[b];
+            for (int b = 0; b < num.n; ++b) nb << (b ? " " : "") << hm.num_count[b];
+            log_msg(kDebug,
+                    "symbolic phase: %d product(s), %lld rows, NIP %llu (max %d per row), nnz(C) %lld (max %d per "
+                    "row), rows per symbolic bin [%s], fail bin %d, rows per fused bin [%s], host %.1f us",
+                    count, total_rows, hm.total_nip, hm.max_nip, all_nnz, hm.max_nnz, sb.str().c_str(), hm.fail_count,
+                    nb.str().c_str(), 1e6 * (t_sym - t_start));
         }
         return SPG_OK;
     }
 
     spg_status launch_numeric() override {
         if (empty) return SPG_OK;
-        const hipStream_t s = L.s;
         // exact-size outputs
         for (int i = 0; i < count; ++i) {
             if (C[i].m == 0) continue;
             C[i].nnz = static_cast<int32_t>(nnz[i]);
             void *pc = nullptr, *pv = nullptr;
-            spg_status st = dev_alloc(c, &pc, sizeof(int32_t) * nnz[i], s);
+            spg_status st = c.cache.alloc(&pc, sizeof(int32_t) * nnz[i]);
             if (st == SPG_OK) owned.push_back(pc);
-            if (st == SPG_OK) st = dev_alloc(c, &pv, sizeof(V) * nnz[i], s);
+            if (st == SPG_OK) st = c.cache.alloc(&pv, sizeof(V) * nnz[i]);
             if (st == SPG_OK) owned.push_back(pv);
             if (st != SPG_OK) return failed(st);
             C[i].col = static_cast<int32_t*>(pc);
@@ -895,32 +1214,53 @@ struct Job : JobBase {
             for (size_t i = 0; i < launches.size(); ++i, ++k) stream_idx[i] = k % Lane::kStreams;
             n_streams = std::min(std::max(k, 1), static_cast<int>(Lane::kStreams));
         }
-        auto stream_of = [&](int q) { return q == 0 ? s : L.aux[q - 1]; };
         if (glob_bin >= 0) {
             const size_t pool_bytes = sizeof(int32_t) * hm.gnum_pool;
             spg_status st = SPG_OK;
             if (graphs) {  // a buffer of the lane, so that its address recurs
-                st = ensure_gpool(L, pool_bytes);
+                st = ensure_gpool(L, pool_bytes, s);
                 num_pool = L.gpool;
             } else {
-                st = dev_alloc(c, &num_pool, pool_bytes, s);
+                st = c.cache.alloc(&num_pool, pool_bytes);
             }
             if (st != SPG_OK) return failed(st);
         }
+        // Eagerly, no stream waits at all: the host has seen the scan stage
+        // complete (the read-back it polled was queued after it), and C and
+        // the pool come from the block cache, not from stream-ordered
+        // allocations.  Each stream ends with a kernel that copies the
+        // flags word and signals the host; an asynchronous call instead
+        // joins the aux streams back to the main stream.  Under graph
+        // capture the aux streams fork from and join the main stream with
+        // events, and one read-back ends the phase.
+        done_mask = 0;
+        if (!mode.events && !mode.async)
+            for (int q = 0; q < n_streams; ++q) {
+                lane_arm(L, q);
+                done_mask |= 1 << q;
+            }
+        if (mode.events) lane_arm(L, 0);
         const spg_status st = queue_phase(1, [&]() -> spg_status {
             hipError_t e = hipSuccess;
+            // A batch reads the descriptors (with C's addresses) from device
+            // memory.  Under graph capture one table is copied on the main
+            // stream and the aux streams fork after it; eagerly every stream
+            // copies its own table before its first launch.
+            ProbSet<V> ps_q[Lane::kStreams];
+            for (int q = 0; q < Lane::kStreams; ++q) {
+                ps_q[q] = ps;
+                if (q > 0 && batch && !mode.events) ps_q[q].many = d_probs + static_cast<size_t>(q) * count;
+            }
             if (batch)
                 e = hipMemcpyAsync(d_probs, L.h_stage + stage_probs, sizeof(Prob<V>) * count, hipMemcpyHostToDevice, s);
-            // C.col, C.val and the pool were allocated on the main stream: an
-            // aux stream waits for that point before its first launch.  The
-            // waits are queued just before those launches, so that the host
-            // time each one takes (several us) is not spent before the first
-            // launch of the main stream, while the GPU idles.
-            if (e == hipSuccess && n_streams > 1) e = hipEventRecord(L.ev_ready, s);
-            bool aux_ready[Lane::kStreams] = {true, false, false, false};
+            if (e == hipSuccess && mode.events && n_streams > 1) e = hipEventRecord(L.ev_ready, s);
+            bool aux_ready[Lane::kStreams] = {true};  // the others false
             auto stream_for = [&](int q) -> hipStream_t {
                 if (!aux_ready[q]) {
-                    if (e == hipSuccess) e = hipStreamWaitEvent(L.aux[q - 1], L.ev_ready, 0);
+                    if (e == hipSuccess && mode.events) e = hipStreamWaitEvent(L.aux[q - 1], L.ev_ready, 0);
+                    if (e == hipSuccess && !mode.events && batch)
+                        e = hipMemcpyAsync(d_probs + static_cast<size_t>(q) * count, L.h_stage + stage_probs,
+                                           sizeof(Prob<V>) * count, hipMemcpyHostToDevice, L.aux[q - 1]);
                     aux_ready[q] = true;
                 }
                 return stream_of(q);
@@ -941,59 +1281,128 @@ struct Job : JobBase {
                 const hipStream_t on = stream_for(stream_idx[li]);
                 if (e != hipSuccess) break;
                 prof.begin("symnum_v" + std::to_string(Lh.variant), Lh.count, grid, on);
-                e = launch_symnum_bin<V>(Lh.variant, ps, items2, Lh.off, Lh.count, meta, grid, on);
+                e = launch_symnum_bin<V>(Lh.variant, ps_q[stream_idx[li]], items2, Lh.off, Lh.count, meta, grid, on);
                 prof.end();
             }
-            for (int i = 1; i < n_streams && e == hipSuccess; ++i) {
-                e = hipEventRecord(L.ev_done[i - 1], L.aux[i - 1]);
-                if (e == hipSuccess) e = hipStreamWaitEvent(s, L.ev_done[i - 1], 0);
+            if (e == hipSuccess) e = hipGetLastError();
+            if (e == hipSuccess && mode.events) {
+                for (int i = 1; i < n_streams && e == hipSuccess; ++i) e = join(L.aux[i - 1], L.ev_done[i - 1], s);
+                // the flags word back and the signal
+                if (e == hipSuccess)
+                    launch_readback(&meta->flags.v, &L.h_done[0].flags, 1, nullptr, nullptr, 0, nullptr,
+                                    &L.h_done[0].flag, s);
+            } else if (e == hipSuccess && mode.async) {
+                for (int i = 1; i < n_streams && e == hipSuccess; ++i) e = join(L.aux[i - 1], L.ev_done[i - 1], s);
+            } else if (e == hipSuccess) {
+                for (int q = 0; q < n_streams; ++q)
+                    launch_readback(&meta->flags.v, &L.h_done[q].flags, 1, nullptr, nullptr, 0, nullptr,
+                                    &L.h_done[q].flag, stream_of(q));
             }
             if (e == hipSuccess) e = hipGetLastError();
-            if (e == hipSuccess) {
-                // the flags word back, the counters zeroed for the next multiply, the signal
-                launch_readback(&meta->flags.v, &L.h_meta->flags.v, 1, nullptr, nullptr, 0, meta, L.h_flag, s);
-                e = hipGetLastError();
-            }
             return e == hipSuccess ? SPG_OK : hip_fail(e, "fused pass", __FILE__, __LINE__);
         });
         if (st != SPG_OK) return failed(st);
+        if (mode.events) done_mask = 1;
         signal_queued = true;
-        if (num_pool && !graphs) {
-            (void)dev_release(c, num_pool, s);
-            num_pool = nullptr;
-        }
         return SPG_OK;
     }
 
     spg_status finish() override {
-        const hipError_t e = signal_queued ? lane_poll(L) : lane_wait(L);
+        if (empty) {
+            owned.clear();
+            return SPG_OK;
+        }
+        hipStream_t streams[Lane::kStreams];
+        for (int q = 0; q < Lane::kStreams; ++q) streams[q] = stream_of(q);
+        if (mode.async) {
+            // C is complete once the main stream reaches this point; the pool
+            // waits for it too
+            if (num_pool && !graphs) (void)c.cache.release(num_pool, s, true);
+            num_pool = nullptr;
+            L.meta_clean = true;  // zeroed by the read-back; a mismatch flag is found by the next call
+            owned.clear();
+            return SPG_OK;
+        }
+        hipError_t e = hipSuccess;
+        if (!signal_queued) {
+            lane_arm(L, 0);
+            launch_signal(&L.h_done[0].flag, s);
+            e = hipGetLastError();
+            done_mask = 1;
+        }
+        if (e == hipSuccess) e = lane_poll(L, done_mask, streams);
         if (e != hipSuccess) return failed(hip_fail(e, "fused pass", __FILE__, __LINE__));
-        if (!empty && (L.h_meta->flags.v & kFlagMismatch))
+        if (num_pool && !graphs) (void)c.cache.release(num_pool, nullptr, false);
+        num_pool = nullptr;
+        int flags = 0;
+        for (int q = 0; q < Lane::kStreams; ++q)
+            if (done_mask >> q & 1) flags |= L.h_done[q].flags;
+        if (flags & kFlagMismatch)
             return failed(fail(SPG_ERR_INTERNAL, "fused pass found a different column count than the count pass "
                                                  "(are B's rows free of duplicate columns and indices in range?)"));
-        if (signal_queued) L.meta_clean = true;  // the last read-back zeroed the counters
-        owned.clear();  // the outputs now belong to the caller
+        L.meta_clean = true;  // zeroed by the symbolic read-back, untouched since
+        owned.clear();        // the outputs now belong to the caller
+        if (log_on(kDebug))
+            log_msg(kDebug, "fused pass done: host %.1f us after the read-back, %.1f us in all",
+                    1e6 * (now_s() - t_sym), 1e6 * (now_s() - t_start));
         return SPG_OK;
     }
 };
 
 template <typename V>
-spg_status multiply_impl(Context& c, int count, const spg_csr* A, const spg_csr* B, spg_csr* C) {
-    prof_reset(c);
-    Job<V> job(c, c.lane, c.sym, c.num, c.profiling);
+spg_status multiply_impl(Context& c, Lane& L, const BinTable& sym, const BinTable& num, bool profile, CallMode mode,
+                         int count, const spg_csr* A, const spg_csr* B, spg_csr* C) {
+    if (profile) prof_reset(c);
+    Job<V> job(c, L, sym, num, profile, mode);
     SPG_TRY(job.symbolic(count, A, B, C));
     SPG_TRY(job.launch_numeric());
     SPG_TRY(job.finish());
-    if (c.profiling) prof_collect(c);
+    if (profile) prof_collect(c);
     return SPG_OK;
 }
 
-// ---------------------------------------------------------------- server
-
-double now_s() {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+// A multiply from a host thread: on its own lane (per caller stream, or per
+// thread for the synchronous calls, whose stream is the thread's default
+// stream) without the library lock; with graph replay or profiling on, on
+// the context's lane under the lock.
+spg_status multiply_entry(int count, const spg_csr* A, const spg_csr* B, spg_csr* C, spg_dtype dtype,
+                          hipStream_t stream, bool async) {
+    if (count < 1 || !A || !B || !C) return fail(SPG_ERR_INVALID_ARGUMENT, "null argument or empty batch");
+    if (dtype != SPG_F32 && dtype != SPG_F64) return fail(SPG_ERR_INVALID_ARGUMENT, "unknown dtype");
+    Context* c = nullptr;
+    SPG_TRY(ensure_context(c));
+    g_last_error.clear();
+    auto run = [&](Lane& L, const BinTable& sym, const BinTable& num, bool profile, CallMode mode) {
+        return dtype == SPG_F64 ? multiply_impl<double>(*c, L, sym, num, profile, mode, count, A, B, C)
+                                : multiply_impl<float>(*c, L, sym, num, profile, mode, count, A, B, C);
+    };
+    {
+        std::unique_lock<std::recursive_mutex> lock(g_mutex);
+        if (c->graphs || c->profiling) {
+            // the context lane's own stream, ordered after the caller's stream
+            CallMode mode{c->lane.own, true, false, false};
+            hipStream_t caller = stream ? stream : hipStreamPerThread;
+            SPG_HIP(hipEventRecord(c->lane.ev_ready, caller));
+            SPG_HIP(hipStreamWaitEvent(c->lane.own, c->lane.ev_ready, 0));
+            const spg_status st = run(c->lane, c->sym, c->num, c->profiling, mode);
+            if (st == SPG_OK && async) SPG_HIP(hipStreamSynchronize(c->lane.own));
+            return st;
+        }
+    }
+    Lane* L = nullptr;
+    SPG_TRY(lane_for(*c, stream, L));
+    BinTable sym, num;
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        sym = c->sym;
+        num = c->num;
+    }
+    std::lock_guard<std::mutex> lk(L->mu);
+    CallMode mode{stream ? stream : hipStreamPerThread, false, false, async};
+    return run(*L, sym, num, false, mode);
 }
 
+// ---------------------------------------------------------------- server
 struct Request {
     int64_t ticket = 0;
     spg_csr A{}, B{};
@@ -1092,14 +1501,15 @@ struct Server {
             f.C.assign(n, spg_csr{});
             BinTable sym, num;
             {
-                std::lock_guard<std::recursive_mutex> g(g_mutex);
+                std::lock_guard<std::mutex> g(c->mu);
                 sym = c->sym;
                 num = c->num;
             }
+            const CallMode mode{L.own, true, false, false};
             if (take.front()->dtype == SPG_F64)
-                f.job = std::make_unique<Job<double>>(*c, L, sym, num, false);
+                f.job = std::make_unique<Job<double>>(*c, L, sym, num, false, mode);
             else
-                f.job = std::make_unique<Job<float>>(*c, L, sym, num, false);
+                f.job = std::make_unique<Job<float>>(*c, L, sym, num, false, mode);
             spg_status st = f.job->symbolic(n, f.A.data(), f.B.data(), f.C.data());
             if (st != SPG_OK) {
                 publish(f, st, g_last_error);
@@ -1158,53 +1568,55 @@ using namespace spg;
 extern "C" {
 
 spg_status spg_init(void) {
-    std::lock_guard<std::recursive_mutex> lock(g_mutex);
     Context* c = nullptr;
     return ensure_context(c);
 }
 
 spg_status spg_multiply(const spg_csr* A, const spg_csr* B, spg_csr* C, spg_dtype dtype) {
-    return spg_multiply_batched(1, A, B, C, dtype);
+    return multiply_entry(1, A, B, C, dtype, nullptr, false);
 }
 
 spg_status spg_multiply_batched(int32_t count, const spg_csr* A, const spg_csr* B, spg_csr* C, spg_dtype dtype) {
-    if (count < 1 || !A || !B || !C) return fail(SPG_ERR_INVALID_ARGUMENT, "null argument or empty batch");
-    if (dtype != SPG_F32 && dtype != SPG_F64) return fail(SPG_ERR_INVALID_ARGUMENT, "unknown dtype");
-    std::lock_guard<std::recursive_mutex> lock(g_mutex);
-    Context* c = nullptr;
-    SPG_TRY(ensure_context(c));
-    g_last_error.clear();
-    return dtype == SPG_F64 ? multiply_impl<double>(*c, count, A, B, C) : multiply_impl<float>(*c, count, A, B, C);
+    return multiply_entry(count, A, B, C, dtype, nullptr, false);
 }
 
-spg_status spg_csr_free(spg_csr* C) {
+spg_status spg_multiply_async(const spg_csr* A, const spg_csr* B, spg_csr* C, spg_dtype dtype, void* stream) {
+    return multiply_entry(1, A, B, C, dtype, static_cast<hipStream_t>(stream), true);
+}
+
+static spg_status csr_free_after(spg_csr* C, hipStream_t after) {
     if (!C) return fail(SPG_ERR_INVALID_ARGUMENT, "null matrix");
-    std::lock_guard<std::recursive_mutex> lock(g_mutex);
     Context* c = nullptr;
     SPG_TRY(ensure_context(c));
-    SPG_TRY(dev_release(*c, C->rpt, c->stream));
-    SPG_TRY(dev_release(*c, C->col, c->stream));
-    SPG_TRY(dev_release(*c, C->val, c->stream));
+    void* blocks[3] = {C->rpt, C->col, C->val};
+    SPG_TRY(c->cache.release_all(blocks, 3, after));
     *C = spg_csr{0, 0, 0, nullptr, nullptr, nullptr};
     return SPG_OK;
+}
+
+spg_status spg_csr_free(spg_csr* C) { return csr_free_after(C, hipStreamPerThread); }
+
+spg_status spg_csr_free_async(spg_csr* C, void* stream) {
+    return csr_free_after(C, stream ? static_cast<hipStream_t>(stream) : hipStreamPerThread);
 }
 
 spg_status spg_get_nip(const spg_csr* A, const spg_csr* B, int64_t* nip) {
     if (!A || !B || !nip) return fail(SPG_ERR_INVALID_ARGUMENT, "null argument");
     SPG_TRY(check_operands(*A, *B));
-    std::lock_guard<std::recursive_mutex> lock(g_mutex);
     Context* c = nullptr;
     SPG_TRY(ensure_context(c));
+    std::lock_guard<std::recursive_mutex> lock(g_mutex);
+    const hipStream_t s = hipStreamPerThread;
     void* d = nullptr;
     SPG_HIP(hipMalloc(&d, sizeof(unsigned long long)));
     unsigned long long h = 0;
-    hipError_t e = hipMemsetAsync(d, 0, sizeof(h), c->stream);
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(h), s);
     if (e == hipSuccess) {
-        launch_total_nip(A->rpt, A->col, B->rpt, A->m, static_cast<unsigned long long*>(d), c->stream);
+        launch_total_nip(A->rpt, A->col, B->rpt, A->m, static_cast<unsigned long long*>(d), s);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(d);
     if (e != hipSuccess) return hip_fail(e, "spg_get_nip", __FILE__, __LINE__);
     *nip = static_cast<int64_t>(h);
@@ -1255,9 +1667,14 @@ const char* spg_status_str(spg_status status) {
 
 const char* spg_last_error(void) { return g_last_error.c_str(); }
 
+spg_status spg_set_log_level(int32_t level) {
+    if (level < kTrace || level > kOff) return fail(SPG_ERR_INVALID_ARGUMENT, "log level must be 0 (trace) .. 6 (off)");
+    g_log_level.store(level);
+    return SPG_OK;
+}
+
 spg_status spg_device_info_get(spg_device_info* info) {
     if (!info) return fail(SPG_ERR_INVALID_ARGUMENT, "null argument");
-    std::lock_guard<std::recursive_mutex> lock(g_mutex);
     Context* c = nullptr;
     SPG_TRY(ensure_context(c));
     std::memset(info, 0, sizeof(*info));
@@ -1295,18 +1712,20 @@ static spg_status parse_table(int32_t phase, const char* text, BinTable& t) {
 
 spg_status spg_set_tuning(int32_t phase, const char* table) {
     if (phase != 0 && phase != 1) return fail(SPG_ERR_INVALID_ARGUMENT, "phase must be 0 or 1");
-    std::lock_guard<std::recursive_mutex> lock(g_mutex);
     Context* c = nullptr;
     SPG_TRY(ensure_context(c));
+    BinTable t;
     if (!table || !*table) {  // reset to the defaults
         BinTable sym, num;
         default_tables(sym, num);
-        (phase == 0 ? c->sym : c->num) = phase == 0 ? sym : num;
-        return SPG_OK;
+        t = phase == 0 ? sym : num;
+    } else {
+        SPG_TRY(parse_table(phase, table, t));
     }
-    BinTable t;
-    SPG_TRY(parse_table(phase, table, t));
+    std::lock_guard<std::recursive_mutex> lock(g_mutex);
+    std::lock_guard<std::mutex> g(c->mu);
     (phase == 0 ? c->sym : c->num) = t;
+    log_msg(kDebug, "%s bins set: %s", phase == 0 ? "symbolic" : "fused", table_text(t).c_str());
     return SPG_OK;
 }
 
@@ -1319,10 +1738,13 @@ static spg_status write_text(const std::string& s, char* buf, size_t len) {
 
 spg_status spg_get_tuning(int32_t phase, char* buf, size_t buf_len) {
     if (phase != 0 && phase != 1) return fail(SPG_ERR_INVALID_ARGUMENT, "phase must be 0 or 1");
-    std::lock_guard<std::recursive_mutex> lock(g_mutex);
     Context* c = nullptr;
     SPG_TRY(ensure_context(c));
-    const BinTable& t = phase == 0 ? c->sym : c->num;
+    BinTable t;
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        t = phase == 0 ? c->sym : c->num;
+    }
     std::ostringstream out;
     for (int b = 0; b < t.n; ++b) out << t.variant[b] << ' ' << t.nip_cap[b] << ' ' << t.nnz_cap[b] << '\n';
     return write_text(out.str(), buf, buf_len);
@@ -1347,15 +1769,14 @@ spg_status spg_get_variants(int32_t phase, char* buf, size_t buf_len) {
 }
 
 spg_status spg_set_graphs(int32_t enabled) {
-    std::lock_guard<std::recursive_mutex> lock(g_mutex);
     Context* c = nullptr;
     SPG_TRY(ensure_context(c));
+    std::lock_guard<std::recursive_mutex> lock(g_mutex);
     c->graphs = enabled != 0;
     return SPG_OK;
 }
 
 spg_status spg_graph_stats(int64_t* captured, int64_t* replayed) {
-    std::lock_guard<std::recursive_mutex> lock(g_mutex);
     Context* c = nullptr;
     SPG_TRY(ensure_context(c));
     if (captured) *captured = c->n_captured.load();
@@ -1364,25 +1785,25 @@ spg_status spg_graph_stats(int64_t* captured, int64_t* replayed) {
 }
 
 spg_status spg_set_profiling(int32_t enabled) {
-    std::lock_guard<std::recursive_mutex> lock(g_mutex);
     Context* c = nullptr;
     SPG_TRY(ensure_context(c));
+    std::lock_guard<std::recursive_mutex> lock(g_mutex);
     c->profiling = enabled != 0;
     prof_reset(*c);
     return SPG_OK;
 }
 
 int32_t spg_profile_count(void) {
-    std::lock_guard<std::recursive_mutex> lock(g_mutex);
     Context* c = nullptr;
     if (ensure_context(c) != SPG_OK) return 0;
+    std::lock_guard<std::recursive_mutex> lock(g_mutex);
     return static_cast<int32_t>(c->prof.size());
 }
 
 spg_status spg_profile_get(int32_t index, char* name, size_t name_len, int64_t* rows, int32_t* grid, double* ms) {
-    std::lock_guard<std::recursive_mutex> lock(g_mutex);
     Context* c = nullptr;
     SPG_TRY(ensure_context(c));
+    std::lock_guard<std::recursive_mutex> lock(g_mutex);
     if (index < 0 || index >= static_cast<int32_t>(c->prof.size()))
         return fail(SPG_ERR_INVALID_ARGUMENT, "profile index out of range");
     const ProfRec& r = c->prof[index];
@@ -1397,10 +1818,7 @@ spg_status spg_server_create(size_t memory_budget_bytes, int32_t max_batch, spg_
     if (!out || max_batch < 1 || memory_budget_bytes == 0) return fail(SPG_ERR_INVALID_ARGUMENT, "bad server arguments");
     *out = nullptr;
     Context* c = nullptr;
-    {
-        std::lock_guard<std::recursive_mutex> lock(g_mutex);
-        SPG_TRY(ensure_context(c));
-    }
+    SPG_TRY(ensure_context(c));
     auto* srv = new Server();
     srv->c = c;
     srv->budget = memory_budget_bytes;
@@ -1417,7 +1835,6 @@ spg_status spg_server_create(size_t memory_budget_bytes, int32_t max_batch, spg_
     *out = reinterpret_cast<spg_server*>(srv);
     return SPG_OK;
 }
-
 spg_status spg_server_submit(spg_server* server, const spg_csr* A, const spg_csr* B, spg_dtype dtype,
                              int64_t* ticket) {
     if (!server || !A || !B || !ticket) return fail(SPG_ERR_INVALID_ARGUMENT, "null argument");

@@ -229,6 +229,28 @@ def random_csr(m: int, n: int, density: float, seed: int, dtype=np.float64,
     return HostCSR(m, n, rpt, col, val)
 
 
+def served_request(k: int, seed: int = 4) -> dict:
+    """Request k of the served workload: value type 70 % FP64 / 30 % FP32,
+    generator drawn uniformly from uniform8 / power-law / stencil / batched
+    instance, rows log-uniform in [2^10, 2^20]."""
+    r = rand_unit(seed, 1, np.arange(4 * k, 4 * k + 4))
+    dtype = np.float64 if r[0] < 0.7 else np.float32
+    kind = ["uniform8", "powerlaw", "stencil", "batched"][min(3, int(r[1] * 4))]
+    m = int(round(2.0 ** (10 + 10 * r[2])))
+    if kind == "uniform8":
+        A = uniform8(m=m, seed=100 + k)
+        B = A
+    elif kind == "powerlaw":
+        A = powerlaw(m=m, seed=100 + k)
+        B = A
+    elif kind == "stencil":
+        A = stencil27(g=max(2, int(round(m ** (1 / 3)))), seed=100 + k)
+        B = A
+    else:
+        A, B = batched_instance(k, seed=seed, m=m)
+    return {"A": A.astype(dtype), "B": B.astype(dtype), "kind": kind, "m": A.m, "dtype": np.dtype(dtype).name}
+
+
 WORKLOADS = {
     "uniform8": lambda: uniform8(),
     "powerlaw": lambda: powerlaw(),

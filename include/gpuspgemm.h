@@ -11,8 +11,17 @@
  *
  *   spg_multiply        proposal::proposal<T>(A, B) -> C
  *                       src/libproposal/include/proposal/proposal.cuh:85-124
+ *   spg_multiply_async  proposal::proposal<T> on the caller's stream: the
+ *                       reference runs on the per-thread default stream
+ *                       (cmake/wrappers.cmake:61-63), so concurrent host
+ *                       threads multiply on streams of their own
  *   spg_csr_free        utils::CSR<T,Device>::~CSR / release()
  *                       src/libutils/include/utils/csr.cuh:114-117, 241-252
+ *   spg_csr_free_async  the same, ordered on a caller stream
+ *   spg_set_log_level   the apps' --loglevel (spdlog level), which the
+ *                       library's debug logs follow: bin tables, nnz(C), the
+ *                       fail-bin size (binning.cuh:181-187, 232-233,
+ *                       sym1.cuh:430)
  *   spg_get_nip         utils::get_nip(a, b) -> int64
  *                       src/libutils/include/utils/nip.cuh:52-78
  *   spg_malloc/spg_free utils::malloc<Device>/free<Device>
@@ -73,8 +82,22 @@ spg_status spg_init(void);
    (B's rows sorted ascending, indices in range).  On success *C holds
    library-allocated device arrays (release with spg_csr_free) and the
    result is complete when the call returns, as the reference's
-   synchronous proposal(). */
+   synchronous proposal().  The work is ordered after what the calling
+   thread queued on its default stream (hipStreamPerThread, which also
+   follows the legacy null stream).  Calls from different host threads run
+   concurrently, each on a lane (aux streams, scratch) of its own. */
 spg_status spg_multiply(const spg_csr *A, const spg_csr *B, spg_csr *C, spg_dtype dtype);
+
+/* C = A . B ordered on `stream` (a hipStream_t; null: the calling thread's
+   default stream).  The call waits once, for the read-back of nnz(C) that
+   sizes C (the method's symbolic phase), and returns once the fused pass is
+   queued: C's arrays are valid at return and its contents are complete when
+   `stream` reaches the point of the call.  Calls on one stream run one
+   after another; calls on different streams run concurrently.  A
+   consistency failure of the fused pass (only possible when B has
+   duplicate columns in a row or indices out of range) is reported by the
+   next call on the same stream. */
+spg_status spg_multiply_async(const spg_csr *A, const spg_csr *B, spg_csr *C, spg_dtype dtype, void *stream);
 
 /* Many independent products C_i = A_i . B_i in one pass of the kernels
    (rows of all instances are binned together).  Arrays have `count`
@@ -82,8 +105,15 @@ spg_status spg_multiply(const spg_csr *A, const spg_csr *B, spg_csr *C, spg_dtyp
 spg_status spg_multiply_batched(int32_t count, const spg_csr *A, const spg_csr *B,
                                 spg_csr *C, spg_dtype dtype);
 
-/* Free the device arrays of a library-allocated matrix and zero it. */
+/* Release the device arrays of a library-allocated matrix and zero it.
+   The library keeps released blocks and hands them out again for later
+   outputs once the GPU work queued so far on the calling thread's default
+   stream has completed; work on other streams that reads C must be ordered
+   before that (as for hipFreeAsync). */
 spg_status spg_csr_free(spg_csr *C);
+/* The same, with the blocks reused once `stream` (a hipStream_t; null:
+   the thread's default stream) reaches this point. */
+spg_status spg_csr_free_async(spg_csr *C, void *stream);
 
 /* Total number of intermediate products sum_i sum_{k in A(i,:)} nnz(B(k,:)). */
 spg_status spg_get_nip(const spg_csr *A, const spg_csr *B, int64_t *nip);
@@ -97,6 +127,13 @@ spg_status spg_device_sync(void);
 const char *spg_status_str(spg_status status);
 /* Message of the last failure on this thread ("" if none). */
 const char *spg_last_error(void);
+
+/* Library log level, spdlog's order: 0 trace (every launch), 1 debug (per
+   multiply: rows, NIP, nnz(C), rows per bin of both phases, fail-bin size,
+   host time of the phases), 2 info ... 6 off (the default).  Before the
+   first call the environment variable SPG_LOG_LEVEL (a name or a number)
+   sets it.  Lines go to stderr. */
+spg_status spg_set_log_level(int32_t level);
 
 /* Device facts used by the tuner and the benchmark. */
 typedef struct spg_device_info {

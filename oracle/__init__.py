@@ -48,6 +48,8 @@ def _load():
     lib.oracle_symbolic.restype = I64
     lib.oracle_numeric.argtypes = [I64, I64, P, P, P, P, P, P, P, P, P, P, ctypes.c_int]
     lib.oracle_numeric.restype = ctypes.c_int
+    lib.oracle_time_product.argtypes = [I64, I64, P, P, P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
+    lib.oracle_time_product.restype = ctypes.c_int
     lib.oracle_hw_threads.argtypes = []
     lib.oracle_hw_threads.restype = ctypes.c_int
     _lib = lib
@@ -112,6 +114,25 @@ def spgemm(A, B, threads: int = 0) -> OracleResult:
     if st != 0:
         raise RuntimeError(f"oracle numeric phase failed ({st})")
     return OracleResult(rpt, col, val, absval, nip)
+
+
+def time_product(A, B, threads: int, warmups: int, runs: int) -> np.ndarray:
+    """Wall-clock seconds of `runs` timed host products (count pass, scan,
+    sorted columns and values into fresh arrays) after `warmups` untimed
+    ones: the CPU baseline of the benchmark."""
+    lib = _load()
+    a_rpt = np.ascontiguousarray(A.rpt, dtype=np.int32)
+    a_col = np.ascontiguousarray(A.col, dtype=np.int32)
+    a_val = np.ascontiguousarray(A.val, dtype=np.float64)
+    b_rpt = np.ascontiguousarray(B.rpt, dtype=np.int32)
+    b_col = np.ascontiguousarray(B.col, dtype=np.int32)
+    b_val = np.ascontiguousarray(B.val, dtype=np.float64)
+    times = np.zeros(max(runs, 1), dtype=np.float64)
+    st = lib.oracle_time_product(A.m, B.n, _ptr(a_rpt), _ptr(a_col), _ptr(a_val), _ptr(b_rpt), _ptr(b_col),
+                                 _ptr(b_val), threads, warmups, runs, _ptr(times))
+    if st != 0:
+        raise MemoryError("host product ran out of memory")
+    return times[:runs]
 
 
 def dense_reference(A, B):

@@ -207,3 +207,36 @@ int oracle_numeric(int64_t m, int64_t n, const int32_t *a_rpt, const int32_t *a_
 }
 
 int oracle_hw_threads(void) { return hw_threads(0); }
+
+/* ---- CPU baseline ------------------------------------------------------
+ * The product alone, as a caller would time it: the count pass and scan
+ * into a fresh row-pointer array, then sorted columns and values into
+ * freshly allocated C arrays (no |A||B|, no NIP), `warmups` untimed runs
+ * and `runs` timed ones on `threads` threads; wall-clock seconds of each
+ * timed run go to times[].  Returns 0, or nonzero when memory runs out. */
+#include <time.h>
+static double now_seconds(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+int oracle_time_product(int64_t m, int64_t n, const int32_t *a_rpt, const int32_t *a_col,
+                        const double *a_val, const int32_t *b_rpt, const int32_t *b_col,
+                        const double *b_val, int threads, int warmups, int runs, double *times) {
+    for (int r = -warmups; r < runs; r++) {
+        const double t0 = now_seconds();
+        int64_t *c_rpt = (int64_t *)malloc((size_t)(m + 1) * sizeof(int64_t));
+        if (!c_rpt) return 1;
+        const int64_t nnz = oracle_symbolic(m, n, a_rpt, a_col, b_rpt, b_col, c_rpt, threads);
+        int32_t *c_col = (int32_t *)malloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(int32_t));
+        double *c_val = (double *)malloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(double));
+        int st = nnz < 0 || !c_col || !c_val;
+        if (!st) st = oracle_numeric(m, n, a_rpt, a_col, a_val, b_rpt, b_col, b_val, c_rpt, c_col, c_val,
+                                     NULL, threads);
+        free(c_rpt); free(c_col); free(c_val);
+        if (st) return 1;
+        if (r >= 0) times[r] = now_seconds() - t0;
+    }
+    return 0;
+}

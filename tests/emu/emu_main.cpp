@@ -101,6 +101,20 @@ static int serve(const std::string& fa, const std::string& fb, int n, long long 
         const double tol = sizeof(T) == 8 ? 1e-12 : 1e-5;
         std::printf("request %d: nnz %d structure %s scaled_err %.3e\n", i, hc.nnz,
                     e.structure ? "identical" : "DIFFERENT", e.scaled);
+        if (!e.structure) {  // where it first differs
+            int r = 0;
+            while (r <= hc.m && hc.rpt[r] == ref.rpt[r]) ++r;
+            int k = 0;
+            while (k < std::min(hc.nnz, ref.nnz) && hc.col[k] == ref.col[k]) ++k;
+            std::printf("  first row pointer differing: %d (%d vs %d); first column differing: %d\n", r,
+                        r <= hc.m ? hc.rpt[r] : -1, r <= ref.m ? ref.rpt[r] : -1, k);
+            std::printf("  columns from there:");
+            for (int t = k; t < std::min(k + 8, hc.nnz); ++t) std::printf(" %d/%d", hc.col[t], ref.col[t]);
+            int bad_cols = 0;
+            for (int t = 0; t < hc.nnz; ++t) bad_cols += hc.col[t] != ref.col[t];
+            std::printf("; %d of %d differ; C at rpt %p col %p val %p\n", bad_cols, hc.nnz, (void*)c.rpt,
+                        (void*)c.col, c.val);
+        }
         if (!e.structure || e.scaled > tol) bad = 1;
     }
     int64_t batches = 0, served = 0, max_seen = 0, deferred = 0, peak = 0;

@@ -5,6 +5,9 @@
 #include <hip/hip_runtime.h>
 
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
+#include <deque>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -147,6 +150,178 @@ Pool& pool() {
 std::vector<Work>*& capture_target() {
     thread_local std::vector<Work>* target = nullptr;
     return target;
+}
+
+// ---- streams
+namespace {
+
+struct Item {
+    enum Kind { kWork, kRecord, kWaitEvent } kind;
+    Work fn;
+    Event ev;                                     // kWaitEvent: the record waited for
+    unsigned long long seq = 0;                   // place in the device's order of queued items
+    const char* label = nullptr;
+};
+
+struct Stream {
+    std::deque<Item> q;
+    unsigned long long done_seq = 0;  // seq of the last item run (stream order)
+    bool blocking = false;            // the legacy and per-thread default streams
+};
+
+std::recursive_mutex g_dev;  // queues, and the device while it runs an item
+// every stream ever created (never destroyed: queued work stays reachable)
+std::vector<Stream*>& g_streams = *new std::vector<Stream*>();
+unsigned long long g_seq = 0;
+Stream* g_legacy = nullptr;
+
+Stream* make_stream(bool blocking) {
+    Stream* s = new Stream();
+    s->blocking = blocking;
+    g_streams.push_back(s);
+    return s;
+}
+
+Stream* resolve(void* h) {  // g_dev held
+    if (h == nullptr) {
+        if (!g_legacy) g_legacy = make_stream(true);
+        return g_legacy;
+    }
+    if (h == reinterpret_cast<void*>(2)) {  // hipStreamPerThread
+        thread_local Stream* mine = nullptr;
+        if (!mine) mine = make_stream(true);
+        return mine;
+    }
+    return static_cast<Stream*>(h);
+}
+
+bool done(const Event& e) {
+    return e.stream == nullptr || static_cast<Stream*>(e.stream)->done_seq >= e.seq;
+}
+
+bool runnable(const Item& it) {
+    switch (it.kind) {
+        case Item::kWaitEvent: return done(it.ev);
+        default: return true;
+    }
+}
+
+// Run one item: the head of `prefer` when it may run (a query or a
+// synchronisation of one stream completes that stream first and leaves the
+// others queued, as a fast stream would on the device), else the runnable
+// stream head queued last.
+bool step(Stream* prefer = nullptr) {
+    Stream* best = nullptr;
+    if (prefer && !prefer->q.empty() && runnable(prefer->q.front())) best = prefer;
+    for (Stream* s : g_streams) {
+        if (best == prefer && best) break;
+        if (!s->q.empty() && runnable(s->q.front()) && (!best || s->q.front().seq > best->q.front().seq)) best = s;
+    }
+    if (!best) return false;
+    Item it = std::move(best->q.front());
+    best->q.pop_front();
+    static const bool trace = std::getenv("EMU_TRACE") != nullptr;
+    if (trace) {
+        int sid = 0;
+        while (g_streams[sid] != best) ++sid;
+        static const char* kinds[] = {"work", "record", "wait-event"};
+        std::fprintf(stderr, "[emu] stream %d item %llu %s %s\n", sid, it.seq, kinds[it.kind],
+                     it.label ? it.label : "");
+    }
+    if (it.kind == Item::kWork) it.fn();
+    best->done_seq = it.seq;
+    return true;
+}
+
+template <typename Pred>
+void drain_until(Pred pred, Stream* prefer = nullptr) {
+    std::lock_guard<std::recursive_mutex> g(g_dev);
+    while (!pred()) {
+        if (!step(prefer)) {
+            std::fprintf(stderr, "emulated streams: a wait can never be satisfied\n");
+            std::abort();
+        }
+    }
+}
+
+void enqueue(void* h, Item it) {
+    std::lock_guard<std::recursive_mutex> g(g_dev);
+    Stream* s = resolve(h);
+    it.seq = ++g_seq;
+    s->q.push_back(std::move(it));
+}
+
+}  // namespace
+
+void* new_stream() {
+    std::lock_guard<std::recursive_mutex> g(g_dev);
+    return make_stream(false);
+}
+
+void submit_on(void* h, Work w, const char* label) {
+    if (std::vector<Work>* r = capture_target()) {
+        r->push_back(std::move(w));
+        return;
+    }
+    Item it{Item::kWork, std::move(w)};
+    it.label = label;
+    enqueue(h, std::move(it));
+}
+
+void record_on(void* h, Event* ev) {
+    if (capture_target()) return;  // a graph replays its work in the recorded order
+    std::lock_guard<std::recursive_mutex> g(g_dev);
+    Stream* s = resolve(h);
+    Item it{Item::kRecord, nullptr};
+    it.seq = ++g_seq;
+    ev->stream = s;
+    ev->seq = it.seq;
+    s->q.push_back(std::move(it));
+}
+
+void wait_event_on(void* h, const Event* ev) {
+    if (capture_target()) return;
+    Item it{Item::kWaitEvent, nullptr};
+    {
+        std::lock_guard<std::recursive_mutex> g(g_dev);
+        it.ev = *ev;  // the record current at the call
+    }
+    enqueue(h, std::move(it));
+}
+
+void drain_stream(void* h) {
+    Stream* s;
+    {
+        std::lock_guard<std::recursive_mutex> g(g_dev);
+        s = resolve(h);
+    }
+    drain_until([s] { return s->q.empty(); }, s);
+}
+
+void drain_event(const Event* ev) {
+    const Event e = *ev;
+    drain_until([e] { return done(e); }, static_cast<Stream*>(e.stream));
+}
+
+bool event_done(const Event* ev) {
+    std::lock_guard<std::recursive_mutex> g(g_dev);
+    return done(*ev);
+}
+
+void drain_all() {
+    drain_until([] {
+        for (Stream* s : g_streams)
+            if (!s->q.empty()) return false;
+        return true;
+    });
+}
+
+void drain_blocking() {
+    drain_until([] {
+        for (Stream* s : g_streams)
+            if (s->blocking && !s->q.empty()) return false;
+        return true;
+    });
 }
 
 int lane() { return static_cast<int>(tl_thread.x % 64); }

@@ -2,8 +2,8 @@
 // Test infrastructure: it lets the library's kernels and host driver be
 // compiled as plain C++ (one OS thread per GPU lane) and run under
 // AddressSanitizer on a machine without a GPU.  Device memory is host heap,
-// LDS arrays are static storage (blocks run one after another), streams and
-// events are synchronous no-ops.
+// LDS arrays are static storage (blocks run one after another), and stream
+// work is queued per stream and run in an adversarial order (emu::submit_on).
 //
 // Cross-lane operations use count-based barriers: a wave barrier releases
 // when every live lane of the wave has arrived at some wave barrier.  The
@@ -60,14 +60,30 @@ void exchange_write(std::uint64_t v);      // publish this lane's value
 std::uint64_t exchange_read(int lane);     // read lane's published value (after a wave barrier)
 void launch(dim3 grid, dim3 block, const std::function<void()>& body);
 int lane();
-// Stream work (launches, copies, fills) runs at once, or is recorded while
-// the calling thread captures a graph and runs when the graph is launched.
+// Streams: work (launches, copies, fills, event records, waits for an
+// event) is queued per stream and runs when the
+// host synchronises or queries (a stream, an event, the device, a
+// synchronous copy or free).  Queued work of different streams runs in the
+// most adversarial order the dependencies allow: among the streams whose
+// first item may run, the item queued last runs first, so work that lacks
+// a stream wait or an event runs before what it should follow.  While the
+// calling thread captures a graph, work is recorded instead and runs when
+// the graph is launched.
 using Work = std::function<void()>;
 std::vector<Work>*& capture_target();
-inline void submit(Work w) {
-    if (std::vector<Work>* r = capture_target()) r->push_back(std::move(w));
-    else w();
-}
+struct Event {
+    void* stream = nullptr;          // stream of the last record (null: never recorded)
+    unsigned long long seq = 0;      // its place in the device's order of queued items
+};
+void submit_on(void* stream, Work w, const char* label = nullptr);
+void record_on(void* stream, Event* ev);
+void wait_event_on(void* stream, const Event* ev);
+void drain_stream(void* stream);
+void drain_event(const Event* ev);
+void drain_all();
+void drain_blocking();  // the legacy and per-thread default streams
+bool event_done(const Event* ev);
+void* new_stream();
 }  // namespace emu
 
 #define threadIdx (::emu::tl_thread)
@@ -242,30 +258,53 @@ inline hipError_t hipGetDevice(int* d) {
 }
 hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int);
 inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
-    static int dummy;
-    *s = reinterpret_cast<hipStream_t>(&dummy);
+    *s = static_cast<hipStream_t>(::emu::new_stream());
     return hipSuccess;
 }
 inline hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) {
-    static int dummy;
-    *e = reinterpret_cast<hipEvent_t>(&dummy);
+    *e = reinterpret_cast<hipEvent_t>(new ::emu::Event());
     return hipSuccess;
 }
 inline hipError_t hipEventCreate(hipEvent_t* e) { return hipEventCreateWithFlags(e, 0); }
-inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+inline ::emu::Event* emu_ev(hipEvent_t e) { return reinterpret_cast<::emu::Event*>(e); }
+inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t s) {
+    ::emu::record_on(s, emu_ev(e));
+    return hipSuccess;
+}
 inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) {
     *ms = 0.f;
     return hipSuccess;
 }
-inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
-inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
-// work runs synchronously in the emulation: every stream and event is complete
-inline hipError_t hipStreamQuery(hipStream_t) { return hipSuccess; }
-inline hipError_t hipEventQuery(hipEvent_t) { return hipSuccess; }
-inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
-inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
-inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
-inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
+inline hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned) {
+    ::emu::wait_event_on(s, emu_ev(e));
+    return hipSuccess;
+}
+inline hipError_t hipStreamSynchronize(hipStream_t s) {
+    ::emu::drain_stream(s);
+    return hipSuccess;
+}
+// the device runs when the host looks: a query completes the queried work
+inline hipError_t hipStreamQuery(hipStream_t s) {
+    ::emu::drain_stream(s);
+    return hipSuccess;
+}
+inline hipError_t hipEventQuery(hipEvent_t e) { return ::emu::event_done(emu_ev(e)) ? hipSuccess : hipErrorNotReady; }
+inline hipError_t hipEventSynchronize(hipEvent_t e) {
+    ::emu::drain_event(emu_ev(e));
+    return hipSuccess;
+}
+inline hipError_t hipStreamDestroy(hipStream_t s) {
+    ::emu::drain_stream(s);
+    return hipSuccess;
+}
+inline hipError_t hipEventDestroy(hipEvent_t e) {
+    delete emu_ev(e);
+    return hipSuccess;
+}
+inline hipError_t hipDeviceSynchronize() {
+    ::emu::drain_all();
+    return hipSuccess;
+}
 inline hipError_t hipDeviceGetDefaultMemPool(hipMemPool_t*, int) { return hipErrorNotSupported; }
 inline hipError_t hipMemPoolSetAttribute(hipMemPool_t, hipMemPoolAttr, void*) { return hipSuccess; }
 inline hipError_t hipMalloc(void** p, std::size_t n) {
@@ -276,26 +315,40 @@ template <typename T>
 inline hipError_t hipMalloc(T** p, std::size_t n) {
     return hipMalloc(reinterpret_cast<void**>(p), n);
 }
-inline hipError_t hipMallocAsync(void** p, std::size_t n, hipStream_t) { return hipMalloc(p, n); }
-inline hipError_t hipFree(void* p) {
+inline hipError_t hipFree(void* p) {  // frees wait for the device, as hipFree does
+    ::emu::drain_all();
     std::free(p);
     return hipSuccess;
 }
-inline hipError_t hipFreeAsync(void* p, hipStream_t) { return hipFree(p); }
+inline hipError_t hipMallocAsync(void** p, std::size_t n, hipStream_t) { return hipMalloc(p, n); }
+inline hipError_t hipFreeAsync(void* p, hipStream_t s) {
+    ::emu::submit_on(s, [p] { std::free(p); });
+    return hipSuccess;
+}
 inline hipError_t hipHostMalloc(void** p, std::size_t n, unsigned) { return hipMalloc(p, n); }
+// the per-thread default stream (a stand-in handle like the real one)
+#define hipStreamPerThread (reinterpret_cast<hipStream_t>(2))
 inline hipError_t hipHostFree(void* p) { return hipFree(p); }
+// synchronous copies and fills run on the legacy stream: after the work of
+// the blocking (default) streams
 inline hipError_t hipMemcpy(void* d, const void* s, std::size_t n, hipMemcpyKind) {
+    ::emu::drain_blocking();
     if (n) std::memcpy(d, s, n);
     return hipSuccess;
 }
-inline hipError_t hipMemcpyAsync(void* d, const void* s, std::size_t n, hipMemcpyKind, hipStream_t) {
-    ::emu::submit([=] {
+inline hipError_t hipMemset(void* d, int v, std::size_t n) {
+    ::emu::drain_blocking();
+    std::memset(d, v, n);
+    return hipSuccess;
+}
+inline hipError_t hipMemcpyAsync(void* d, const void* s, std::size_t n, hipMemcpyKind, hipStream_t st) {
+    ::emu::submit_on(st, [=] {
         if (n) std::memcpy(d, s, n);
     });
     return hipSuccess;
 }
-inline hipError_t hipMemsetAsync(void* d, int v, std::size_t n, hipStream_t) {
-    ::emu::submit([=] { std::memset(d, v, n); });
+inline hipError_t hipMemsetAsync(void* d, int v, std::size_t n, hipStream_t st) {
+    ::emu::submit_on(st, [=] { std::memset(d, v, n); });
     return hipSuccess;
 }
 
@@ -320,8 +373,10 @@ inline hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, void*, vo
     *e = new emu_graph_exec{*g};
     return hipSuccess;
 }
-inline hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
-    for (auto& w : e->work) w();
+inline hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t s) {
+    ::emu::submit_on(s, [e] {
+        for (auto& w : e->work) w();
+    });
     return hipSuccess;
 }
 inline hipError_t hipGraphDestroy(hipGraph_t g) {
@@ -339,6 +394,9 @@ inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, F, int, s
 }
 
 #define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...)                           \
-    ::emu::submit([=, emu_grid_ = dim3(grid), emu_block_ = dim3(block)]() {                     \
-        ::emu::launch(emu_grid_, emu_block_, [&]() { kernel(__VA_ARGS__); });                   \
-    })
+    ::emu::submit_on(                                                                         \
+        stream,                                                                               \
+        [=, emu_grid_ = dim3(grid), emu_block_ = dim3(block)]() {                             \
+            ::emu::launch(emu_grid_, emu_block_, [&]() { kernel(__VA_ARGS__); });             \
+        },                                                                                    \
+        #kernel)

@@ -49,9 +49,12 @@ def test_powerlaw_small(gpu_lib, dtype):
     compare(A, A, gpu_multiply(A, A))
 
 
-def test_powerlaw_full_f64(gpu_lib):
-    A = gen.powerlaw()
-    _say("powerlaw m=1e6 f64")
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_powerlaw_full(gpu_lib, dtype):
+    """The full 10^6-row power-law product in both value types (FP32 operands
+    are the narrowed FP64 ones; the oracle runs in FP64 on them)."""
+    A = gen.powerlaw().astype(dtype)
+    _say("powerlaw m=1e6", np.dtype(dtype).name)
     st = compare(A, A, gpu_multiply(A, A))
     _say(st)
 

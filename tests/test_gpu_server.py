@@ -111,3 +111,37 @@ def test_plain_calls_beside_a_server(gpu_lib):
     finally:
         dA.free()
         dU.free()
+
+
+def test_served_workload_sample(gpu_lib):
+    """The first 32 requests of the served workload, drawn exactly as the
+    benchmark draws them (all four generators, FP64 and FP32, rows up to
+    2^20), submitted to a server configured like the benchmark's: every
+    served result has the structure of its plain result and both match the
+    oracle."""
+    import oracle
+    reqs = [gen.served_request(k) for k in range(32)]
+    kinds = {q["kind"] for q in reqs}
+    assert kinds == {"uniform8", "powerlaw", "stencil", "batched"}
+    assert {q["dtype"] for q in reqs} == {"float64", "float32"}
+    assert max(q["m"] for q in reqs) > 800_000
+    dev = [(q["A"].to_device(), q["B"].to_device()) for q in reqs]
+    try:
+        alone = [_plain(dA, dB) for dA, dB in dev]
+        with g.Server(budget_bytes=16 << 30, max_batch=32) as srv:
+            tickets = [srv.submit(dA, dB) for dA, dB in dev]
+            served = []
+            for t in tickets:
+                dC, _ = srv.wait(t)
+                served.append(dC.to_host())
+                dC.free()
+        for q, a, s in zip(reqs, alone, served):
+            print(q["kind"], q["m"], q["dtype"], flush=True)
+            assert np.array_equal(a.rpt, s.rpt) and np.array_equal(a.col, s.col)
+            ref = oracle.spgemm(q["A"], q["B"])
+            compare(q["A"], q["B"], a, ref)
+            compare(q["A"], q["B"], s, ref)
+    finally:
+        for dA, dB in dev:
+            dA.free()
+            dB.free()
