@@ -1007,6 +1007,9 @@ struct Job : JobBase {
         const size_t o_tsum = take(sizeof(long long) * n_tiles);
         const size_t o_inst = take(sizeof(long long) * count);
         const size_t o_nip = take(sizeof(int32_t) * M);
+        size_t nnz_a = 0;
+        for (int i = 0; i < count; ++i) nnz_a += static_cast<size_t>(A[i].nnz);
+        const size_t o_bref = take(sizeof(long long) * nnz_a);
         const size_t o_it1 = take(sizeof(int2) * M * sym.n);  // a region of M items per symbolic bin
         const size_t o_it2 = take(sizeof(RowItem) * M);
         const size_t o_fit = take(sizeof(int2) * M);
@@ -1025,6 +1028,7 @@ struct Job : JobBase {
         tile_sums = reinterpret_cast<long long*>(base + o_tsum);
         inst_nnz = reinterpret_cast<long long*>(base + o_inst);
         int32_t* nip = reinterpret_cast<int32_t*>(base + o_nip);
+        long long* bref = reinterpret_cast<long long*>(base + o_bref);
         items1 = reinterpret_cast<int2*>(base + o_it1);
         items2 = reinterpret_cast<RowItem*>(base + o_it2);
         fail_items = reinterpret_cast<int2*>(base + o_fit);
@@ -1036,6 +1040,7 @@ struct Job : JobBase {
         // descriptors
         probs.assign(count, Prob<V>{});
         long long row_base = 0;
+        size_t entry_base = 0;
         for (int i = 0; i < count; ++i) {
             Prob<V>& p = probs[i];
             p.a_rpt = A[i].rpt;
@@ -1048,6 +1053,7 @@ struct Job : JobBase {
             p.c_col = nullptr;
             p.c_val = nullptr;
             p.nip = nip + row_base;
+            p.bref = bref + entry_base;
             p.m = A[i].m;
             p.n = B[i].n;
             if (batch) {
@@ -1056,6 +1062,7 @@ struct Job : JobBase {
                     tiles.push_back(Tile{i, r0, std::min(kTileRows, A[i].m - r0), first});
             }
             row_base += A[i].m;
+            entry_base += static_cast<size_t>(A[i].nnz);
         }
         ps.p0 = probs[0];
         ps.n_inst = count;

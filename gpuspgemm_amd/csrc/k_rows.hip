@@ -40,6 +40,29 @@ __global__ __launch_bounds__(kRowThreads) void k_nip(ProbSet<V> ps, BinTable sym
 
     const Tile t = ps.tile(blockIdx.x);
     const GProb<V> P = ps.get(t.inst);
+    // The row of B (start, length) of every entry of the tile's rows is kept
+    // for the later passes.  In a tile of short rows (at most four entries
+    // on average, as in power-law matrices) each thread stores its rows'
+    // entries as it reads them below; in a tile of longer rows the threads
+    // first walk the tile's entries side by side (coalesced loads of A's
+    // columns and coalesced stores, four loads of each in flight): one row
+    // per thread would scatter the stores of a long row over many lines.
+    const int te0 = P.a_rpt[t.row0], te1 = P.a_rpt[t.row0 + t.rows];
+    const bool by_row = te1 - te0 <= 4 * t.rows;
+    if (!by_row) {
+        constexpr int kU = 4;
+        for (int base = te0 + threadIdx.x; base < te1; base += kU * kRowThreads) {
+            int kk[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) kk[u] = base + u * kRowThreads < te1 ? P.a_col[base + u * kRowThreads] : -1;
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+                if (kk[u] >= 0) {
+                    const int b0 = P.b_rpt[kk[u]], b1 = P.b_rpt[kk[u] + 1];
+                    P.bref[base + u * kRowThreads] = pack_bref(b0, b1 - b0);
+                }
+        }
+    }
     constexpr int kPer = kTileRows / kRowThreads;
     constexpr int kShort = 4;  // entries of a row read without a loop
     // all loads of a thread's rows are issued together: row pointers, then
@@ -65,7 +88,11 @@ __global__ __launch_bounds__(kRowThreads) void k_nip(ProbSet<V> ps, BinTable sym
         s[j] = 0;
 #pragma unroll
         for (int q = 0; q < kShort; ++q)
-            if (k[j][q] >= 0) s[j] += P.b_rpt[k[j][q] + 1] - P.b_rpt[k[j][q]];
+            if (k[j][q] >= 0) {
+                const int b0 = P.b_rpt[k[j][q]], b1 = P.b_rpt[k[j][q] + 1];
+                s[j] += b1 - b0;
+                if (by_row) P.bref[a0[j] + q] = pack_bref(b0, b1 - b0);
+            }
     }
     int my_max = 0;
     long long my_total = 0;
@@ -84,7 +111,11 @@ __global__ __launch_bounds__(kRowThreads) void k_nip(ProbSet<V> ps, BinTable sym
                 for (int q = 0; q < 8; ++q) kk[q] = base + q < a1[j] ? P.a_col[base + q] : -1;
 #pragma unroll
                 for (int q = 0; q < 8; ++q)
-                    if (kk[q] >= 0) s[j] += P.b_rpt[kk[q] + 1] - P.b_rpt[kk[q]];
+                    if (kk[q] >= 0) {
+                        const int b0 = P.b_rpt[kk[q]], b1 = P.b_rpt[kk[q] + 1];
+                        s[j] += b1 - b0;
+                        if (by_row) P.bref[base + q] = pack_bref(b0, b1 - b0);
+                    }
             }
         unsigned long long longs = __ballot(len > 32);
         while (longs) {
@@ -101,7 +132,11 @@ __global__ __launch_bounds__(kRowThreads) void k_nip(ProbSet<V> ps, BinTable sym
                 for (int q = 0; q < 8; ++q) kk[q] = base + q * kWave < la1 ? P.a_col[base + q * kWave] : -1;
 #pragma unroll
                 for (int q = 0; q < 8; ++q)
-                    if (kk[q] >= 0) part += P.b_rpt[kk[q] + 1] - P.b_rpt[kk[q]];
+                    if (kk[q] >= 0) {
+                        const int b0 = P.b_rpt[kk[q]], b1 = P.b_rpt[kk[q] + 1];
+                        part += b1 - b0;
+                        if (by_row) P.bref[base + q * kWave] = pack_bref(b0, b1 - b0);
+                    }
             }
             part = wave_sum64(part);
             if (lane_id() == src) s[j] = part;

@@ -36,7 +36,10 @@ constexpr int kMultiThreads = 256; // workgroup size of the multi-bin kernels
 
 // One product C = A . B.  `nip` is per-row scratch; c_rpt first holds nnz
 // per row and then the row pointers (the reference reuses C.rpt the same
-// way, proposal.cuh:40-56).
+// way, proposal.cuh:40-56).  `bref` is per-entry scratch of A: the start
+// and length of the row of B that entry e of A selects, recorded by the NIP
+// pass (which reads those row pointers anyway), so that the later passes go
+// from A's entries straight to B's columns, one dependent load fewer.
 template <typename V>
 struct Prob {
     const int32_t* a_rpt;
@@ -49,9 +52,18 @@ struct Prob {
     int32_t* c_col;
     V* c_val;
     int32_t* nip;
+    long long* bref;
     int32_t m;
     int32_t n;
 };
+
+// An entry's row of B packed in one word: start in the low half, length in
+// the high half.
+__host__ __device__ __forceinline__ long long pack_bref(int start, int len) {
+    return static_cast<long long>(static_cast<unsigned int>(start)) | (static_cast<long long>(len) << 32);
+}
+__host__ __device__ __forceinline__ int bref_start(long long r) { return static_cast<int>(r); }
+__host__ __device__ __forceinline__ int bref_len(long long r) { return static_cast<int>(r >> 32); }
 
 // Global address space.  Data pointers reach the kernels inside structs, so
 // the compiler sees them as generic and would emit flat instructions, which
@@ -80,6 +92,7 @@ struct GProb {
     int32_t SPG_GLOBAL* c_col;
     V SPG_GLOBAL* c_val;
     int32_t SPG_GLOBAL* nip;
+    long long SPG_GLOBAL* bref;
     int32_t m;
     int32_t n;
 };
@@ -89,7 +102,8 @@ template <typename V, typename R>
 __device__ __forceinline__ GProb<V> global_view(R&& p) {
     return GProb<V>{to_global(p.a_rpt), to_global(p.a_col), to_global(p.a_val), to_global(p.b_rpt),
                     to_global(p.b_col), to_global(p.b_val), to_global(p.c_rpt), to_global(p.c_col),
-                    to_global(p.c_val), to_global(p.nip),   p.m,                p.n};
+                    to_global(p.c_val), to_global(p.nip),   to_global(p.bref),  p.m,
+                    p.n};
 }
 
 // Tile of a row-parallel pass: instance, first row, row count, index of the

@@ -81,7 +81,8 @@ __device__ __forceinline__ int group_excl_scan(int v, int lr, int* s_tmp, int& t
 }
 
 // Metadata of one chunk of EC entries of A's row, per row group (in LDS for
-// L > 1, in registers for a lane that owns its row).
+// L > 1, in registers for a lane that owns its row).  Entry e's row of B
+// comes from P.bref[e] (start, length), recorded by the NIP pass.
 template <typename V, int EC, bool VALS>
 struct alignas(8) ChunkMeta {
     // L != 64: exclusive prefix of B-row lengths, entries past the chunk
@@ -93,6 +94,14 @@ struct alignas(8) ChunkMeta {
     int d[EC];
     V av[VALS ? EC : 1];     // a_ik
 };
+
+// Row of B selected by entry e of A, as the NIP pass recorded it.
+template <typename V>
+__device__ __forceinline__ void entry_row(const GProb<V>& P, int e, int& bs, int& len) {
+    const long long r = P.bref[e];
+    bs = bref_start(r);
+    len = bref_len(r);
+}
 
 // Load entries [ebase, min(ebase + EC, a1)) of A's row into `cm`; returns
 // the chunk's product count.  Each lane loads EC/L consecutive entries.  A
@@ -114,10 +123,8 @@ __device__ __forceinline__ int load_chunk(const GProb<V>& P, int ebase, int a1, 
         bs[t] = 0;
         av[t] = V(0);
         if (e < a1) {
-            const int k = P.a_col[e];
             if constexpr (VALS) av[t] = P.a_val[e];
-            bs[t] = P.b_rpt[k];
-            len[t] = P.b_rpt[k + 1] - bs[t];
+            entry_row<V>(P, e, bs[t], len[t]);
         }
         local += len[t];
         filled += len[t] > 0 ? 1 : 0;
@@ -234,10 +241,8 @@ __device__ __forceinline__ int load_chunk_lane(const GProb<V>& P, int ebase, int
         len[t] = 0;
         if constexpr (VALS) cm.av[t] = V(0);
         if (e < a1) {
-            const int k = P.a_col[e];
             if constexpr (VALS) cm.av[t] = P.a_val[e];
-            bs[t] = P.b_rpt[k];
-            len[t] = P.b_rpt[k + 1] - bs[t];
+            entry_row<V>(P, e, bs[t], len[t]);
         }
     }
 #pragma unroll
