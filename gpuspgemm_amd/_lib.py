@@ -50,6 +50,8 @@ _SIG = {
                                     ctypes.POINTER(SpgCsr), _I32]),
     "spg_multiply_async": (_I32, [ctypes.POINTER(SpgCsr), ctypes.POINTER(SpgCsr), ctypes.POINTER(SpgCsr), _I32,
                                   _P]),
+    "spg_multiply_noalloc": (_I32, [ctypes.POINTER(SpgCsr), ctypes.POINTER(SpgCsr), ctypes.POINTER(SpgCsr),
+                                    ctypes.c_int64, _I32]),
     "spg_csr_free": (_I32, [ctypes.POINTER(SpgCsr)]),
     "spg_csr_free_async": (_I32, [ctypes.POINTER(SpgCsr), _P]),
     "spg_set_log_level": (_I32, [_I32]),

@@ -47,6 +47,15 @@
 namespace spg {
 namespace {
 
+// Scratch of spg_multiply_noalloc: a fixed 256 MiB device array that every
+// GPU the library runs on holds until the process ends.  Host code never
+// passes `scratch` itself to a kernel or a copy (in host code the name
+// stands for a host-side shadow, not a device address): each device's
+// context looks the array's address on that device up once, at set-up,
+// with hipGetSymbolAddress, and uses only that address.
+constexpr size_t kNoAllocScratch = size_t(256) << 20;
+static __device__ __align__(256) char scratch[kNoAllocScratch];
+
 thread_local std::string g_last_error;
 
 spg_status fail(spg_status s, const std::string& msg) {
@@ -442,6 +451,8 @@ struct Context {
     int occ_sym_multi[2] = {1, 1}, occ_sym_max = 1;
     int occ_num[2][kNumLdsVariants] = {};
     Lane lane;  // graph replay and profiling run here, one call at a time
+    Lane noalloc_lane;             // spg_multiply_noalloc, one call at a time
+    char* noalloc_base = nullptr;  // this device's address of `scratch`
     std::unordered_map<uint64_t, std::unique_ptr<Lane>> lanes;  // per caller stream
     std::vector<std::unique_ptr<Lane>> free_lanes;              // lanes of threads that ended
     BlockCache cache;
@@ -561,6 +572,9 @@ spg_status ensure_context(Context*& out) {
         c.lds < kSymVariants[kSymMaxVariant].t * 4 + 8 * kSymVariants[kSymMaxVariant].ec + 1024)
         return fail(SPG_ERR_UNSUPPORTED_DEVICE, "device offers less LDS per workgroup than the kernels need");
     SPG_TRY(lane_create(c.lane));
+    SPG_TRY(lane_create(c.noalloc_lane));
+    // the fixed scratch of spg_multiply_noalloc on this device
+    SPG_HIP(hipGetSymbolAddress(reinterpret_cast<void**>(&c.noalloc_base), HIP_SYMBOL(scratch)));
     default_tables(c.sym, c.num);
     SPG_TRY(check_sym_table(c.sym));  // a stale tuned_tables.inc fails here, loudly
     SPG_TRY(check_num_table(c.num));
@@ -734,6 +748,12 @@ struct CallMode {
     bool events = false;         // fused pass forked and joined with events, one read-back at the end (graph capture)
     bool async = false;          // return once the fused pass is queued; C is ready in `main`'s order
                                  // (single products only: no staged descriptors outlive the call)
+    // No allocation at all (spg_multiply_noalloc): scratch and the global
+    // tables come from `fixed` (fixed_cap bytes), C's arrays from the
+    // caller (out_cap entries of C.col / C.val).
+    char* fixed = nullptr;
+    size_t fixed_cap = 0;
+    long long out_cap = 0;
 };
 
 // One batch of products on one lane, in two phases:
@@ -800,6 +820,9 @@ struct Job : JobBase {
     bool graphs = false;        // phases may be replayed from the lane's graph caches
     bool signal_queued = false; // the fused pass already queued its completion signals
     int done_mask = 0;          // streams whose completion words the fused pass fills
+    size_t scratch_end = 0;     // bytes of the scratch layout
+    int32_t* out_col = nullptr; // the caller's C.col / C.val (spg_multiply_noalloc)
+    void* out_val = nullptr;
     double t_start = 0, t_sym = 0;
 
     // Only the context's own lane captures graphs: its calls run under the
@@ -820,9 +843,16 @@ struct Job : JobBase {
         // kernels queued before the failure may still use the blocks
         if (!owned.empty()) (void)c.cache.release_all(owned.data(), static_cast<int>(owned.size()), s);
         owned.clear();
-        if (num_pool && !graphs) (void)c.cache.release(num_pool, s, true);
+        if (num_pool && !graphs && !mode.fixed) (void)c.cache.release(num_pool, s, true);
         num_pool = nullptr;
         L.meta_clean = false;
+        if (mode.fixed) {
+            // the caller's arrays stay with the caller; wait for anything
+            // queued on them before returning
+            (void)hipStreamSynchronize(s);
+            for (hipStream_t a : L.aux) (void)hipStreamSynchronize(a);
+            return;
+        }
         for (int i = 0; i < count; ++i) C[i] = spg_csr{0, 0, 0, nullptr, nullptr, nullptr};
     }
 
@@ -968,17 +998,24 @@ struct Job : JobBase {
         count = n;
         C = out;
         batch = count > 1;
+        if (mode.fixed) {  // the caller's output arrays (a single product)
+            out_col = C[0].col;
+            out_val = C[0].val;
+        }
         for (int i = 0; i < count; ++i) {
             SPG_TRY(check_operands(A[i], B[i]));
-            C[i] = spg_csr{0, A[i].m, B[i].n, nullptr, nullptr, nullptr};
+            C[i] = mode.fixed ? spg_csr{0, A[i].m, B[i].n, C[i].rpt, C[i].col, C[i].val}
+                              : spg_csr{0, A[i].m, B[i].n, nullptr, nullptr, nullptr};
         }
         // output row pointers
         for (int i = 0; i < count; ++i) {
-            void* p = nullptr;
-            const spg_status st = c.cache.alloc(&p, sizeof(int32_t) * (static_cast<size_t>(A[i].m) + 1));
-            if (st != SPG_OK) return failed(st);
-            owned.push_back(p);
-            C[i].rpt = static_cast<int32_t*>(p);
+            if (!mode.fixed) {
+                void* p = nullptr;
+                const spg_status st = c.cache.alloc(&p, sizeof(int32_t) * (static_cast<size_t>(A[i].m) + 1));
+                if (st != SPG_OK) return failed(st);
+                owned.push_back(p);
+                C[i].rpt = static_cast<int32_t*>(p);
+            }
             total_rows += A[i].m;
             n_tiles += (A[i].m + kTileRows - 1) / kTileRows;
             if (A[i].m == 0) {
@@ -1019,11 +1056,19 @@ struct Job : JobBase {
         // its own stream (the eager fused pass has no stream waits)
         const size_t o_probs = take(batch ? sizeof(Prob<V>) * count * Lane::kStreams : 0);
         const size_t o_tiles = take(batch ? sizeof(Tile) * n_tiles : 0);
-        {
+        scratch_end = off;
+        char* base = nullptr;
+        if (mode.fixed) {
+            if (off > mode.fixed_cap)
+                return failed(fail(SPG_ERR_OUT_OF_MEMORY,
+                                   "the product needs " + std::to_string(off) + " bytes of scratch; the fixed scratch "
+                                   "of spg_multiply_noalloc holds " + std::to_string(mode.fixed_cap)));
+            base = mode.fixed;
+        } else {
             const spg_status st = ensure_scratch(L, off, s);
             if (st != SPG_OK) return failed(st);
+            base = L.scratch;
         }
-        char* base = L.scratch;
         meta = reinterpret_cast<DevMeta*>(base + o_meta);
         tile_sums = reinterpret_cast<long long*>(base + o_tsum);
         inst_nnz = reinterpret_cast<long long*>(base + o_inst);
@@ -1108,7 +1153,16 @@ struct Job : JobBase {
             const int fails = L.h_meta->fail_count.v;
             hm.fail_count = fails;
             void* sym_pool = nullptr;
-            st = c.cache.alloc(&sym_pool, sizeof(int32_t) * L.h_meta->fail_pool.v);
+            const size_t pool_bytes = sizeof(int32_t) * L.h_meta->fail_pool.v;
+            if (mode.fixed) {  // after the scratch layout, in the fixed array
+                if (scratch_end + pool_bytes > mode.fixed_cap)
+                    st = fail(SPG_ERR_OUT_OF_MEMORY, "the global count tables of the heaviest rows do not fit in the "
+                                                     "fixed scratch of spg_multiply_noalloc");
+                else
+                    sym_pool = mode.fixed + scratch_end;
+            } else {
+                st = c.cache.alloc(&sym_pool, pool_bytes);
+            }
             if (st == SPG_OK) {
                 prof.begin("sym_global", fails, fails, s);
                 launch_sym_global<V>(ps, fail_items, fail_off, static_cast<int*>(sym_pool), fails, s);
@@ -1123,7 +1177,7 @@ struct Job : JobBase {
             }
             // the main stream has completed (or failed): the pool is idle, or
             // waits for the stream
-            if (sym_pool) (void)c.cache.release(sym_pool, s, st != SPG_OK);
+            if (sym_pool && !mode.fixed) (void)c.cache.release(sym_pool, s, st != SPG_OK);
         }
         if (st != SPG_OK) return failed(st);
         hm.flags = L.h_meta->flags.v;
@@ -1163,7 +1217,18 @@ struct Job : JobBase {
     spg_status launch_numeric() override {
         if (empty) return SPG_OK;
         // exact-size outputs
-        for (int i = 0; i < count; ++i) {
+        if (mode.fixed) {  // the caller's arrays (a single product)
+            C[0].nnz = static_cast<int32_t>(nnz[0]);
+            if (nnz[0] > mode.out_cap)
+                return failed(fail(SPG_ERR_OUT_OF_MEMORY, "nnz(C) = " + std::to_string(nnz[0]) +
+                                                              " exceeds the output capacity " +
+                                                              std::to_string(mode.out_cap)));
+            C[0].col = out_col;
+            C[0].val = out_val;
+            probs[0].c_col = out_col;
+            probs[0].c_val = static_cast<V*>(out_val);
+        }
+        for (int i = 0; i < count && !mode.fixed; ++i) {
             if (C[i].m == 0) continue;
             C[i].nnz = static_cast<int32_t>(nnz[i]);
             void *pc = nullptr, *pv = nullptr;
@@ -1227,6 +1292,12 @@ struct Job : JobBase {
             if (graphs) {  // a buffer of the lane, so that its address recurs
                 st = ensure_gpool(L, pool_bytes, s);
                 num_pool = L.gpool;
+            } else if (mode.fixed) {  // after the scratch layout, in the fixed array
+                if (scratch_end + pool_bytes > mode.fixed_cap)
+                    st = fail(SPG_ERR_OUT_OF_MEMORY, "the global tables of the heaviest rows do not fit in the fixed "
+                                                     "scratch of spg_multiply_noalloc");
+                else
+                    num_pool = mode.fixed + scratch_end;
             } else {
                 st = c.cache.alloc(&num_pool, pool_bytes);
             }
@@ -1324,7 +1395,7 @@ struct Job : JobBase {
         if (mode.async) {
             // C is complete once the main stream reaches this point; the pool
             // waits for it too
-            if (num_pool && !graphs) (void)c.cache.release(num_pool, s, true);
+            if (num_pool && !graphs && !mode.fixed) (void)c.cache.release(num_pool, s, true);
             num_pool = nullptr;
             L.meta_clean = true;  // zeroed by the read-back; a mismatch flag is found by the next call
             owned.clear();
@@ -1339,7 +1410,7 @@ struct Job : JobBase {
         }
         if (e == hipSuccess) e = lane_poll(L, done_mask, streams);
         if (e != hipSuccess) return failed(hip_fail(e, "fused pass", __FILE__, __LINE__));
-        if (num_pool && !graphs) (void)c.cache.release(num_pool, nullptr, false);
+        if (num_pool && !graphs && !mode.fixed) (void)c.cache.release(num_pool, nullptr, false);
         num_pool = nullptr;
         int flags = 0;
         for (int q = 0; q < Lane::kStreams; ++q)
@@ -1407,6 +1478,37 @@ spg_status multiply_entry(int count, const spg_csr* A, const spg_csr* B, spg_csr
     std::lock_guard<std::mutex> lk(L->mu);
     CallMode mode{stream ? stream : hipStreamPerThread, false, false, async};
     return run(*L, sym, num, false, mode);
+}
+
+// A single product into caller-provided C arrays with the device's fixed
+// scratch array: no device or pinned allocation inside the call.  Calls on
+// one device take turns on the no-alloc lane; the array keeps the last
+// call's contents, and the job clears the counters that must start at zero.
+spg_status multiply_noalloc(const spg_csr* A, const spg_csr* B, spg_csr* C, int64_t capacity, spg_dtype dtype) {
+    if (!A || !B || !C) return fail(SPG_ERR_INVALID_ARGUMENT, "null argument");
+    if (dtype != SPG_F32 && dtype != SPG_F64) return fail(SPG_ERR_INVALID_ARGUMENT, "unknown dtype");
+    if (capacity < 0) return fail(SPG_ERR_INVALID_ARGUMENT, "negative output capacity");
+    if (!C->rpt) return fail(SPG_ERR_INVALID_ARGUMENT, "C->rpt must hold A->m + 1 row pointers");
+    if (capacity > 0 && (!C->col || !C->val))
+        return fail(SPG_ERR_INVALID_ARGUMENT, "C->col and C->val must hold `capacity` entries");
+    Context* c = nullptr;
+    SPG_TRY(ensure_context(c));
+    g_last_error.clear();
+    if (!c->noalloc_base) return fail(SPG_ERR_INTERNAL, "the fixed scratch of this device has no address");
+    BinTable sym, num;
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        sym = c->sym;
+        num = c->num;
+    }
+    Lane& L = c->noalloc_lane;
+    std::lock_guard<std::mutex> lk(L.mu);
+    CallMode mode{hipStreamPerThread, false, false, false};
+    mode.fixed = c->noalloc_base;
+    mode.fixed_cap = kNoAllocScratch;
+    mode.out_cap = capacity;
+    return dtype == SPG_F64 ? multiply_impl<double>(*c, L, sym, num, false, mode, 1, A, B, C)
+                            : multiply_impl<float>(*c, L, sym, num, false, mode, 1, A, B, C);
 }
 
 // ---------------------------------------------------------------- server
@@ -1589,6 +1691,10 @@ spg_status spg_multiply_batched(int32_t count, const spg_csr* A, const spg_csr* 
 
 spg_status spg_multiply_async(const spg_csr* A, const spg_csr* B, spg_csr* C, spg_dtype dtype, void* stream) {
     return multiply_entry(1, A, B, C, dtype, static_cast<hipStream_t>(stream), true);
+}
+
+spg_status spg_multiply_noalloc(const spg_csr* A, const spg_csr* B, spg_csr* C, int64_t capacity, spg_dtype dtype) {
+    return spg::multiply_noalloc(A, B, C, capacity, dtype);
 }
 
 static spg_status csr_free_after(spg_csr* C, hipStream_t after) {

@@ -99,6 +99,22 @@ spg_status spg_multiply(const spg_csr *A, const spg_csr *B, spg_csr *C, spg_dtyp
    next call on the same stream. */
 spg_status spg_multiply_async(const spg_csr *A, const spg_csr *B, spg_csr *C, spg_dtype dtype, void *stream);
 
+/* No counterpart in the reference: C = A . B for callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving
+   loops, harnesses that forbid hipMalloc).  The caller passes C->rpt
+   (A->m + 1 entries) and C->col / C->val (`capacity` entries each); the
+   call fills them, sets C->nnz, C->m, C->n and returns once C is complete.
+   Scratch comes from a fixed 256 MiB device array that the library holds
+   on each device until the process ends; no device or pinned memory is
+   allocated inside the call.  A product whose scratch does not fit in it
+   fails with SPG_ERR_OUT_OF_MEMORY before any launch; nnz(C) > capacity
+   fails with SPG_ERR_OUT_OF_MEMORY and C->nnz set to the nnz needed (C's
+   arrays then hold no result).  Calls on one device share the array: they
+   take turns on a lock, so calls that would overlap run one after
+   another.  Work is ordered after the calling thread's default stream. */
+spg_status spg_multiply_noalloc(const spg_csr *A, const spg_csr *B, spg_csr *C, int64_t capacity,
+                                spg_dtype dtype);
+
 /* Many independent products C_i = A_i . B_i in one pass of the kernels
    (rows of all instances are binned together).  Arrays have `count`
    entries; each C[i] is allocated like spg_multiply's. */

@@ -1,14 +1,15 @@
 // Driver of the CPU emulation build: C = A . B through the library's C ABI
 // (kernels emulated on the host), compared with the host Gustavson product.
 //   emu_spgemm A.csr B.csr [--f32] [--sym-table FILE] [--num-table FILE] [--repeat N]
-//              [--graphs] [--serve N BUDGET_BYTES MAX_BATCH]
+//              [--graphs] [--serve N BUDGET_BYTES MAX_BATCH] [--noalloc]
 // Exit status 0 when the structure is identical and values agree to 1e-12
 // (FP64) / 1e-5 (FP32) after |A||B| scaling.  --serve submits the product N
 // times to a server (spg_server_*) and checks every result; a budget below
 // one output must fail every request with SPG_ERR_OUT_OF_MEMORY.
 // --graphs turns graph replay on; repeat r then multiplies a fresh device
 // copy of A with its values scaled by r + 1 (new addresses, new values,
-// same launches), and the graph counters are printed.
+// same launches), and the graph counters are printed.  --noalloc runs the
+// products through spg_multiply_noalloc into caller arrays.
 #include <cstdio>
 #include <fstream>
 #include <sstream>
@@ -54,6 +55,49 @@ static int run(const std::string& fa, const std::string& fb, int repeat, bool gr
         utils::handle_spg_error(spg_graph_stats(&captured, &replayed), "graph_stats");
         std::printf("graphs captured %lld replayed %lld\n", (long long)captured, (long long)replayed);
     }
+    return bad;
+}
+
+// spg_multiply_noalloc into caller arrays: a capacity one short of nnz(C)
+// must fail with SPG_ERR_OUT_OF_MEMORY and report the nnz needed; then
+// `repeat` products at the exact capacity, each compared with the host.
+template <typename T>
+static int noalloc(const std::string& fa, const std::string& fb, int repeat) {
+    const auto ha = utils::HostCSR<T>::load_from_filename(fa);
+    const auto hb = utils::HostCSR<T>::load_from_filename(fb);
+    const auto da = ha.template to<utils::Location::Device>();
+    const auto db = hb.template to<utils::Location::Device>();
+    const auto ref = host::gustavson(ha, hb);
+    const auto absref = host::abs_product(ha, hb);
+    const spg_dtype dt = sizeof(T) == 8 ? SPG_F64 : SPG_F32;
+    const spg_csr A = da.raw(), B = db.raw();
+    void *rpt = nullptr, *col = nullptr, *val = nullptr;
+    utils::handle_spg_error(spg_malloc(&rpt, sizeof(int32_t) * (ha.m + 1)), "malloc");
+    utils::handle_spg_error(spg_malloc(&col, sizeof(int32_t) * (ref.nnz + 1)), "malloc");
+    utils::handle_spg_error(spg_malloc(&val, sizeof(T) * (ref.nnz + 1)), "malloc");
+    int bad = 0;
+    if (ref.nnz > 0) {
+        spg_csr c{0, 0, 0, static_cast<int32_t*>(rpt), static_cast<int32_t*>(col), val};
+        const spg_status st = spg_multiply_noalloc(&A, &B, &c, ref.nnz - 1, dt);
+        std::printf("capacity %d: status %s, nnz %d\n", ref.nnz - 1, spg_status_str(st), c.nnz);
+        if (st != SPG_ERR_OUT_OF_MEMORY || c.nnz != ref.nnz) bad = 1;
+    }
+    for (int r = 0; r < repeat; ++r) {
+        spg_csr c{0, 0, 0, static_cast<int32_t*>(rpt), static_cast<int32_t*>(col), val};
+        utils::handle_spg_error(spg_multiply_noalloc(&A, &B, &c, ref.nnz, dt), "multiply_noalloc");
+        utils::HostCSR<T> hc(c.nnz, c.m, c.n);
+        utils::handle_spg_error(spg_memcpy(hc.rpt, c.rpt, sizeof(int32_t) * (c.m + 1)), "memcpy");
+        utils::handle_spg_error(spg_memcpy(hc.col, c.col, sizeof(int32_t) * c.nnz), "memcpy");
+        utils::handle_spg_error(spg_memcpy(hc.val, c.val, sizeof(T) * c.nnz), "memcpy");
+        const auto e = host::errors(hc, ref, absref);
+        const double tol = sizeof(T) == 8 ? 1e-12 : 1e-5;
+        std::printf("noalloc run %d: nnz %d (ref %d) structure %s scaled_err %.3e\n", r, hc.nnz, ref.nnz,
+                    e.structure ? "identical" : "DIFFERENT", e.scaled);
+        if (!e.structure || e.scaled > tol) bad = 1;
+    }
+    utils::handle_spg_error(spg_free(rpt), "free");
+    utils::handle_spg_error(spg_free(col), "free");
+    utils::handle_spg_error(spg_free(val), "free");
     return bad;
 }
 
@@ -131,7 +175,7 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "usage: %s A.csr B.csr [--f32] [--sym-table F] [--num-table F] [--repeat N]\n", argv[0]);
         return 2;
     }
-    bool f32 = false, graphs = false;
+    bool f32 = false, graphs = false, use_noalloc = false;
     int repeat = 1, serve_n = 0, max_batch = 1;
     long long budget = 0;
     for (int i = 3; i < argc; ++i) {
@@ -143,6 +187,8 @@ int main(int argc, char** argv) {
             utils::handle_spg_error(spg_set_tuning(a == "--sym-table" ? 0 : 1, t.c_str()), "set_tuning");
         } else if (a == "--graphs") {
             graphs = true;
+        } else if (a == "--noalloc") {
+            use_noalloc = true;
         } else if (a == "--repeat" && i + 1 < argc) {
             repeat = std::atoi(argv[++i]);
         } else if (a == "--serve" && i + 3 < argc) {
@@ -159,6 +205,8 @@ int main(int argc, char** argv) {
         if (serve_n > 0)
             return f32 ? serve<float>(argv[1], argv[2], serve_n, budget, max_batch)
                        : serve<double>(argv[1], argv[2], serve_n, budget, max_batch);
+        if (use_noalloc)
+            return f32 ? noalloc<float>(argv[1], argv[2], repeat) : noalloc<double>(argv[1], argv[2], repeat);
         return f32 ? run<float>(argv[1], argv[2], repeat, graphs) : run<double>(argv[1], argv[2], repeat, graphs);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "error: %s\n", e.what());

@@ -336,6 +336,13 @@ inline hipError_t hipMemcpy(void* d, const void* s, std::size_t n, hipMemcpyKind
     if (n) std::memcpy(d, s, n);
     return hipSuccess;
 }
+// host threads share one address space: a __device__ variable's address is
+// the host object's address
+#define HIP_SYMBOL(x) (&(x))
+inline hipError_t hipGetSymbolAddress(void** p, const void* symbol) {
+    *p = const_cast<void*>(symbol);
+    return hipSuccess;
+}
 inline hipError_t hipMemset(void* d, int v, std::size_t n) {
     ::emu::drain_blocking();
     std::memset(d, v, n);

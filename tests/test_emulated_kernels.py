@@ -163,10 +163,9 @@ def test_wave_rows_count_entry_starts(emu, tmp_path, phase, variant):
     run_emu(emu, tmp_path, A.astype(np.float32), B.astype(np.float32), "--f32", tables=table)
 
 
-def test_global_paths(emu, tmp_path):
+def _heavy_row_case():
     """A row that overflows the largest LDS table in the count pass and every
-    LDS capacity in the fused pass, plus ordinary rows, first with the
-    default tables and then with every row forced to the global kernels."""
+    LDS capacity in the fused pass, plus three ordinary rows."""
     nb = 20_000
     r0 = np.unique(gen.rand_below(3, 1, np.arange(9500), np.full(9500, nb)))
     rows = [r0] + [np.unique(gen.rand_below(3, 3 + i, np.arange(3), np.full(3, nb))) for i in range(3)]
@@ -179,9 +178,35 @@ def test_global_paths(emu, tmp_path):
     np.cumsum(np.bincount(keys // 200_000, minlength=nb), out=brpt[1:])
     bcol = (keys % 200_000).astype(np.int32)
     B = HostCSR(nb, 200_000, brpt.astype(np.int32), bcol, gen.rand_unit(4, 2, np.arange(bcol.size)) * 2 - 1)
+    return A, B
+
+
+GLOBAL_TABLES = {0: f"{SYM_MAX} {INT_MAX} {INT_MAX}\n", 1: f"{NUM_GLOBAL} {INT_MAX} {INT_MAX}\n"}
+
+
+def test_global_paths(emu, tmp_path):
+    """The heavy-row case first with the default tables and then with every
+    row forced to the global kernels."""
+    A, B = _heavy_row_case()
     run_emu(emu, tmp_path, A, B)
-    run_emu(emu, tmp_path, A, B,
-            tables={0: f"{SYM_MAX} {INT_MAX} {INT_MAX}\n", 1: f"{NUM_GLOBAL} {INT_MAX} {INT_MAX}\n"})
+    run_emu(emu, tmp_path, A, B, tables=GLOBAL_TABLES)
+
+
+def test_noalloc_entry(emu, tmp_path):
+    """spg_multiply_noalloc into caller arrays, scratch in the fixed device
+    array: a capacity one short of nnz(C) fails with out-of-memory and
+    reports the nnz needed, then repeated products match the host; the
+    heavy-row case forced to the global kernels carves both global tables
+    out of the fixed array too."""
+    P = gen.powerlaw(m=1500, seed=5)
+    out = run_emu(emu, tmp_path, P, P, "--noalloc", "--repeat", "2")
+    assert out.count("structure identical") == 2
+    assert "status out of device memory" in out
+    S = gen.stencil27(g=5).astype(np.float32)
+    run_emu(emu, tmp_path, S, S, "--f32", "--noalloc")
+    A, B = _heavy_row_case()
+    out = run_emu(emu, tmp_path, A, B, "--noalloc", "--repeat", "2", tables=GLOBAL_TABLES)
+    assert out.count("structure identical") == 2
 
 
 def test_server_batches_and_admission(emu, tmp_path):
