@@ -67,17 +67,24 @@ def classify(table: str, nip: np.ndarray, nnz: np.ndarray) -> tuple[np.ndarray, 
     return out, [v for v, _, _ in bins]
 
 
-def kernel_bytes(name: str, A, B, C_rpt: np.ndarray, s: int, sym_table: str, num_table: str) -> int:
+def kernel_bytes(name: str, A, B, C_rpt: np.ndarray, s: int, sym_table: str, num_table: str,
+                 merged: dict = None) -> int:
     """Algorithmic (compulsory) bytes of one launch of kernel `name`: the A
     rows it handles (row pointers, columns, values when it reads them),
-    every B row those rows reference, read once, and what it writes."""
+    every B row those rows reference, read once, and what it writes.
+    `merged` maps each merged launch set to its fused variants (the bins
+    of launch symnum_multi<set>)."""
     nip = row_nip(A, B)
     nnz_c = np.diff(C_rpt.astype(np.int64))
     arow = np.diff(A.rpt.astype(np.int64))
-    if name.startswith("symnum_v"):
-        variant = int(name[len("symnum_v"):])
+    if name.startswith("symnum_v") or name.startswith("symnum_multi"):
         b, variants = classify(num_table, nip, nnz_c)
-        sel = (nnz_c > 0) & np.isin(b, [i for i, v in enumerate(variants) if v == variant])
+        if name.startswith("symnum_multi"):
+            members = (merged or {}).get(int(name[len("symnum_multi"):]), ())
+            bins = [i for i, v in enumerate(variants) if v in members]
+        else:
+            bins = [i for i, v in enumerate(variants) if v == int(name[len("symnum_v"):])]
+        sel = (nnz_c > 0) & np.isin(b, bins)
         val_a = val_b = s
         write = (4 + s) * int(nnz_c[sel].sum())
     elif name == "sym_multi":
@@ -138,10 +145,14 @@ def run_gpu(A, steps: int, warmup: int, pause_ms: float, dtype) -> dict:
             for r in prof.collect():
                 kernel_ms.setdefault(r["name"], []).append(r["ms"])
     sym_table, num_table = g.get_tuning(0), g.get_tuning(1)
+    merged = {}
+    for v in g.variant_list(1):
+        if v["merged"] is not None:
+            merged.setdefault(v["merged"], []).append(v["id"])
     dA.free()
     dB.free()
     return {"times": np.array(times), "nip": nip, "c_rpt": c_rpt, "c_nnz": c_nnz, "kernel_ms": kernel_ms,
-            "sym_table": sym_table, "num_table": num_table, "info": g.device_info()}
+            "sym_table": sym_table, "num_table": num_table, "merged": merged, "info": g.device_info()}
 
 
 def summarize(A, res: dict, dtype_name: str, workload: str, cfg: dict) -> dict:
@@ -154,7 +165,7 @@ def summarize(A, res: dict, dtype_name: str, workload: str, cfg: dict) -> dict:
     totals = {k: float(np.sum(v)) for k, v in res["kernel_ms"].items()}
     dom = max(totals, key=totals.get)
     dom_avg_ms = float(np.mean(res["kernel_ms"][dom]))
-    dom_bytes = kernel_bytes(dom, A, A, res["c_rpt"], s, res["sym_table"], res["num_table"])
+    dom_bytes = kernel_bytes(dom, A, A, res["c_rpt"], s, res["sym_table"], res["num_table"], res["merged"])
     achieved = dom_bytes / (dom_avg_ms * 1e-3) / 1e9 if dom_bytes else None
     traffic = traffic_record(workload, dtype_name, dom)
     return {

@@ -125,11 +125,12 @@ def variant_list(phase: int) -> list[dict]:
     """Compiled kernel variants of a phase as dicts: id, wg, rows, lanes
     (per row), table (phase 0) or capacity (phase 1), chunk, slots, and the
     flags max (phase 0's overflow-detecting variant) / global (phase 1's
-    global-memory variant)."""
+    global-memory variant), and merged: the merged launch set (0 small
+    rows, 1 longer rows) whose bins share one launch, or None."""
     out = []
     for line in get_variants(phase).splitlines():
         parts = line.split()
-        v = {"id": int(parts[0]), "max": "max" in parts[1:], "global": "global" in parts[1:]}
+        v = {"id": int(parts[0]), "max": "max" in parts[1:], "global": "global" in parts[1:], "merged": None}
         for kv in parts[1:]:
             if "=" in kv:
                 k, x = kv.split("=")

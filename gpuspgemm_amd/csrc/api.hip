@@ -1245,46 +1245,77 @@ struct Job : JobBase {
         ps.p0 = probs[0];
         if (batch) std::memcpy(L.h_stage + stage_probs, probs.data(), sizeof(Prob<V>) * count);
 
-        // fused pass: one launch per non-empty bin, heaviest bins first,
-        // spread over the lane's streams so that small bins overlap
+        // fused pass: one launch per non-empty bin, or per merged set of
+        // bins, heaviest first, spread over the lane's streams so that
+        // launches overlap
         struct Launch {
             int variant, off, count;
         };
         std::vector<Launch> launches;
         int glob_bin = -1, glob_off = 0;
+        NumMultiBins mb[2] = {};  // bins of the two merged sets
+        int set_units[2] = {0, 0};
         {
             int item_off = 0;
             for (int b = 0; b < num.n; ++b) {
                 const int cnt = hm.num_count[b];
                 const int v = num.variant[b];
+                const int set = num_multi_set(v);
                 if (cnt > 0) {
                     if (v == kNumGlobalVariant) {
                         glob_bin = b;
                         glob_off = item_off;
+                    } else if (set >= 0) {
+                        NumMultiBins& m = mb[set];
+                        const int units = (cnt + kNumVariants[v].r - 1) / kNumVariants[v].r;
+                        m.variant[m.n] = v;
+                        m.off[m.n] = item_off;
+                        m.count[m.n] = cnt;
+                        m.slice0[m.n + 1] = m.slice0[m.n] + (units + kXcds - 1) / kXcds;
+                        ++m.n;
+                        set_units[set] += units;
                     } else {
                         launches.push_back(Launch{v, item_off, cnt});
                     }
                 }
                 item_off += cnt;
             }
+            for (int set = 0; set < 2; ++set) {
+                NumMultiBins& m = mb[set];
+                // one bin, or set 1 holding a large bin: launches of their own
+                if (m.n == 1 || (set == 1 && set_units[1] > kNumMultiMaxUnits1)) {
+                    for (int i = 0; i < m.n; ++i) launches.push_back(Launch{m.variant[i], m.off[i], m.count[i]});
+                    m.n = 0;
+                }
+            }
         }
-        // Stream plan: launches in order of decreasing row capacity times
-        // lanes per row, dealt round robin over the lane's streams.  The
-        // heavy-row launches (few workgroups, each a long chain) are queued
-        // first, so their workgroups reach the CUs before the launches of
-        // many small rows fill them, and then run beside those.  (Measured
-        // against a longest-first plan and a plan that keeps the
-        // GPU-filling launches on one stream: this one was fastest.)
+        // Stream plan: the single launches in order of decreasing row
+        // capacity times lanes per row, dealt round robin over the lane's
+        // streams but one per merged launch.  The heavy-row launches (few
+        // workgroups, each a long chain) are queued first, so their
+        // workgroups reach the CUs before the merged launches fill them, and
+        // then run beside them; set 1 (longer rows) is queued before set 0.
+        // (Measured against a longest-first plan: this one was faster.)
         std::stable_sort(launches.begin(), launches.end(), [](const Launch& x, const Launch& y) {
             return kNumVariants[x.variant].p * kNumVariants[x.variant].wg / kNumVariants[x.variant].r >
                    kNumVariants[y.variant].p * kNumVariants[y.variant].wg / kNumVariants[y.variant].r;
         });
+        const int merged_sets[2] = {1, 0};  // queue order
+        int n_merged = 0;
+        for (int set : merged_sets) n_merged += mb[set].n > 0 ? 1 : 0;
         std::vector<int> stream_idx(launches.size());
         int n_streams = 1;
+        int merged_q[2] = {-1, -1};
+        size_t merged_at = 0;  // the merged launches are queued before single launch merged_at
         {
+            const int n_deal = Lane::kStreams - n_merged;
             int k = glob_bin >= 0 ? 1 : 0;  // the global bin, when present, is queued first on the main stream
-            for (size_t i = 0; i < launches.size(); ++i, ++k) stream_idx[i] = k % Lane::kStreams;
-            n_streams = std::min(std::max(k, 1), static_cast<int>(Lane::kStreams));
+            for (size_t i = 0; i < launches.size(); ++i, ++k) stream_idx[i] = k % n_deal;
+            int q = std::min(k, n_deal);
+            for (int set : merged_sets)
+                if (mb[set].n > 0) merged_q[set] = q++;
+            merged_at = std::min(launches.size(), static_cast<size_t>(n_deal));
+            n_streams = std::max(q, 1);
         }
         if (glob_bin >= 0) {
             const size_t pool_bytes = sizeof(int32_t) * hm.gnum_pool;
@@ -1350,7 +1381,20 @@ struct Job : JobBase {
                                         static_cast<int*>(num_pool), meta, on);
                 prof.end();
             }
+            auto queue_merged = [&]() {
+                for (int set : merged_sets) {
+                    if (e != hipSuccess || mb[set].n == 0) continue;
+                    const hipStream_t on = stream_for(merged_q[set]);
+                    if (e != hipSuccess) return;
+                    int rows = 0;
+                    for (int b = 0; b < mb[set].n; ++b) rows += mb[set].count[b];
+                    prof.begin("symnum_multi" + std::to_string(set), rows, kXcds * mb[set].slice0[mb[set].n], on);
+                    e = launch_symnum_multi<V>(set, ps_q[merged_q[set]], items2, mb[set], meta, on);
+                    prof.end();
+                }
+            };
             for (size_t li = 0; li < launches.size(); ++li) {
+                if (li == merged_at) queue_merged();
                 if (e != hipSuccess) break;
                 const Launch& Lh = launches[li];
                 const NumVariant& nv = kNumVariants[Lh.variant];
@@ -1362,6 +1406,7 @@ struct Job : JobBase {
                 e = launch_symnum_bin<V>(Lh.variant, ps_q[stream_idx[li]], items2, Lh.off, Lh.count, meta, grid, on);
                 prof.end();
             }
+            if (merged_at == launches.size()) queue_merged();
             if (e == hipSuccess) e = hipGetLastError();
             if (e == hipSuccess && mode.events) {
                 for (int i = 1; i < n_streams && e == hipSuccess; ++i) e = join(L.aux[i - 1], L.ev_done[i - 1], s);
@@ -1873,7 +1918,8 @@ spg_status spg_get_variants(int32_t phase, char* buf, size_t buf_len) {
     } else if (phase == 1) {
         for (int v = 0; v < kNumGlobalVariant; ++v)
             out << v << " wg=" << kNumVariants[v].wg << " rows=" << kNumVariants[v].r << " capacity="
-                << kNumVariants[v].p << " chunk=" << kNumVariants[v].ec << " slots=" << kNumVariants[v].pmax << '\n';
+                << kNumVariants[v].p << " chunk=" << kNumVariants[v].ec << " slots=" << kNumVariants[v].pmax
+                << (num_multi_set(v) >= 0 ? " merged=" + std::to_string(num_multi_set(v)) : std::string()) << '\n';
         out << kNumGlobalVariant << " global\n";
     } else {
         return fail(SPG_ERR_INVALID_ARGUMENT, "phase must be 0 or 1");

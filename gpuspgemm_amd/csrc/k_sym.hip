@@ -75,7 +75,7 @@ constexpr int sym_multi_lds() {
 // a kernel takes the register budget of its largest variant, and theirs is
 // far smaller, so twice as many of their waves fit on a CU.
 template <typename V, int LO, int HI>
-__global__ __launch_bounds__(kMultiThreads) void k_sym_multi(ProbSet<V> ps, const int2* __restrict__ items,
+__global__ __launch_bounds__(kMultiThreads) SPG_WAVES_PER_EU(kSymMultiMinWaves[LO == 0 ? 0 : 1]) void k_sym_multi(ProbSet<V> ps, const int2* __restrict__ items,
                                                              int bin_cap, BinTable sym, int n_multi,
                                                              const DevMeta* __restrict__ meta) {
     static_assert(sym_multi_lds<LO, HI>() <= kSymMultiLds, "symbolic multi-bin LDS budget");

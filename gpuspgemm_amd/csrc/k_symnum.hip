@@ -175,7 +175,7 @@ __device__ __forceinline__ void symnum_rows(const ProbSet<V>& ps, const RowItem*
 // unit of a workgroup is chosen so that consecutive units stay on one XCD.
 // A smaller grid walks the units persistently.
 template <typename V, int VV>
-__global__ __launch_bounds__(kNumVariants[VV].wg) void k_symnum_bin(ProbSet<V> ps, const RowItem* __restrict__ items,
+__global__ __launch_bounds__(kNumVariants[VV].wg) SPG_WAVES_PER_EU(kNumMinWaves[VV]) void k_symnum_bin(ProbSet<V> ps, const RowItem* __restrict__ items,
                                                                     int item_off, int count, DevMeta* meta) {
     constexpr NumVariant kV = kNumVariants[VV];
     __shared__ __align__(16) int smem[kV.r * num_row_ints(kV)];
@@ -185,6 +185,45 @@ __global__ __launch_bounds__(kNumVariants[VV].wg) void k_symnum_bin(ProbSet<V> p
     for (int u = ur.first; u < ur.end; u += ur.stride)
         symnum_rows<V, kV.wg, kV.r, kV.p, kV.ec, kV.pmax>(ps, items, item_off + u * kV.r, item_off + count, smem,
                                                           s_tmp, meta);
+}
+
+#define SPG_NUM_VARIANTS(X) \
+    X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18)
+
+// Several bins of the fused pass in one launch, a workgroup per unit.
+// Each XCD gets a contiguous slice of every bin (balanced work across the
+// XCDs, and neighbouring rows, which read the same rows of B, meet in one
+// L2).  Registers are the largest member's (kept within the set's
+// residency floor), LDS the largest member's; a workgroup runs one unit, so
+// no LDS layout is reused by another variant.
+template <typename V, int SET>
+__global__ __launch_bounds__(256) SPG_WAVES_PER_EU(kNumMultiMinWaves[SET]) void k_symnum_multi(
+    ProbSet<V> ps, const RowItem* __restrict__ items, NumMultiBins mb, DevMeta* meta) {
+    __shared__ __align__(16) int smem[max_num_multi_lds(SET) / 4];
+    __shared__ int s_tmp[256 / kWave];
+    const int xcd = blockIdx.x % kXcds;
+    const int local = blockIdx.x / kXcds;
+    int b = 0;
+    while (b + 1 < mb.n && local >= mb.slice0[b + 1]) ++b;
+    const int per = mb.slice0[b + 1] - mb.slice0[b];
+    const int k = xcd * per + (local - mb.slice0[b]);  // unit of bin b
+    const int end = mb.off[b] + mb.count[b];
+#define SPG_MULTI_CASE(v)                                                                                   \
+    case v:                                                                                                 \
+        if constexpr (num_multi_set(v) == SET) {                                                            \
+            static_assert(kNumVariants[v].wg == 256, "merged launches run 256-thread variants");           \
+            if (mb.off[b] + k * kNumVariants[v].r < end)                                                    \
+                symnum_rows<V, 256, kNumVariants[v].r, kNumVariants[v].p, kNumVariants[v].ec,               \
+                            kNumVariants[v].pmax>(ps, items, mb.off[b] + k * kNumVariants[v].r, end, smem, s_tmp, \
+                                                  meta);                                                    \
+        }                                                                                                   \
+        break;
+    switch (mb.variant[b]) {
+        SPG_NUM_VARIANTS(SPG_MULTI_CASE)
+        default:
+            break;
+    }
+#undef SPG_MULTI_CASE
 }
 
 // Lower bound in a sorted global list (values read coherently).
@@ -320,8 +359,6 @@ static int occupancy_variant() {
     return n > 0 ? n : 1;
 }
 
-#define SPG_NUM_VARIANTS(X) \
-    X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18)
 
 template <typename V>
 hipError_t launch_symnum_bin(int variant, const ProbSet<V>& ps, const RowItem* items, int item_off, int count,
@@ -335,6 +372,19 @@ hipError_t launch_symnum_bin(int variant, const ProbSet<V>& ps, const RowItem* i
             return hipErrorInvalidValue;
     }
 #undef SPG_LAUNCH_CASE
+}
+
+template <typename V>
+hipError_t launch_symnum_multi(int set, const ProbSet<V>& ps, const RowItem* items, const NumMultiBins& mb,
+                               DevMeta* meta, hipStream_t s) {
+    const int grid = kXcds * mb.slice0[mb.n];  // one unit per workgroup
+    if (set == 0)
+        hipLaunchKernelGGL((k_symnum_multi<V, 0>), dim3(grid), dim3(256), 0, s, ps, items, mb, meta);
+    else if (set == 1)
+        hipLaunchKernelGGL((k_symnum_multi<V, 1>), dim3(grid), dim3(256), 0, s, ps, items, mb, meta);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 template <typename V>
@@ -360,6 +410,8 @@ int occupancy_symnum_bin(int variant) {
 #define SPG_INSTANTIATE_NUM(V)                                                                               \
     template hipError_t launch_symnum_bin<V>(int, const ProbSet<V>&, const RowItem*, int, int, DevMeta*, int,    \
                                              hipStream_t);                                                    \
+    template hipError_t launch_symnum_multi<V>(int, const ProbSet<V>&, const RowItem*, const NumMultiBins&,       \
+                                               DevMeta*, hipStream_t);                                        \
     template void launch_symnum_global<V>(const ProbSet<V>&, const RowItem*, const long long*, int, int, int*, \
                                           DevMeta*, hipStream_t);                                             \
     template int occupancy_symnum_bin<V>(int);

@@ -74,6 +74,11 @@ __host__ __device__ __forceinline__ int bref_len(long long r) { return static_ca
 #define SPG_GLOBAL __attribute__((address_space(1)))
 #endif
 
+// Minimum waves per SIMD a kernel must keep: caps its VGPRs at 512 / n.
+#ifndef SPG_WAVES_PER_EU
+#define SPG_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
+#endif
+
 template <typename T>
 __device__ __forceinline__ T SPG_GLOBAL* to_global(T* p) {
     return (T SPG_GLOBAL*)p;

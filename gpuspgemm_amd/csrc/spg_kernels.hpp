@@ -32,6 +32,9 @@ constexpr int kSymSmallVariants = 4;  // variants 0..3 run in a multi-bin launch
 constexpr int kSymMaxVariant = 9;
 constexpr int kSymMaxThreshold = 26214;  // distinct keys admitted in the 32768-slot table (80 %)
 constexpr int kSymMultiLds = 36 * 1024;  // bytes: max over variants 0..8 of r * sym_row_ints * 4
+// Waves per SIMD kept by the two multi-bin launches (small tables, larger
+// tables), as for kNumMinWaves.
+constexpr int kSymMultiMinWaves[2] = {7, 4};
 
 // Fused sorted-columns + numeric variants, one kernel each: all use 256
 // threads except 9 (a 1024-thread workgroup per row); rows above every
@@ -47,7 +50,50 @@ constexpr NumVariant kNumVariants[] = {
     {256, 32, 16, 8, 4},    {256, 64, 16, 8, 8},   {256, 32, 32, 16, 8},
 };
 constexpr int kNumLdsVariants = 19;  // variants with an LDS kernel (0..18)
+// Waves per SIMD that each fused variant's kernel must keep (its register
+// budget is 512 / waves VGPRs).  Fetching a lane's products as one batch
+// of loads (every load in flight before the first multiply) needs more
+// registers than fetching them one by one; the floor keeps the residency
+// the variant had with the one-by-one fetch, and the compiler schedules
+// the batch within it.  Measured on gfx950: the 96^3 stencil's fused pass
+// 3.23 -> 2.81 ms, the power-law's 198k-row bin 70 -> 53 us.
+constexpr int kNumMinWaves[kNumLdsVariants] = {3, 3, 3, 3, 3, 4, 3, 4, 3, 4, 8, 8, 7, 6, 7, 5, 7, 5, 5};
 constexpr int kNumGlobalVariant = 19;
+
+// Fused variants whose bins share one launch.  Set 0, small rows (256
+// threads, up to 32 lanes per row, about 64 VGPRs): their bins hold most
+// rows of skewed matrices, and one launch keeps the GPU filled across them
+// where separate launches would queue behind each other on the lane's few
+// hardware queues.  Set 1, the 256-thread variants of longer rows: merged
+// only while their bins are small (a few thousand units, as the heavy tail
+// of a skewed matrix), so that those long-latency units all start at once
+// beside set 0 instead of queueing one launch after another; a large bin
+// of set 1 (the 96^3 stencil's) keeps its own launch and its own register
+// budget.  kNumMultiMinWaves is each merged kernel's residency floor.
+constexpr int kNumMultiSet0[] = {10, 11, 12, 13, 14, 16};
+constexpr int kNumMultiSet1[] = {4, 5, 6, 7, 8};
+constexpr int kNumMultiMinWaves[2] = {6, 3};
+constexpr int kNumMultiMaxUnits1 = 4096;  // set 1 merges while its bins hold at most this many units
+constexpr int num_multi_set(int v) {
+    for (int m : kNumMultiSet0)
+        if (m == v) return 0;
+    for (int m : kNumMultiSet1)
+        if (m == v) return 1;
+    return -1;
+}
+
+// Bins of one merged launch: bin b's rows are items [off[b], off[b] +
+// count[b]) in units of its variant's rows.  Every XCD takes one contiguous
+// slice of every bin: the workgroups with local index (blockIdx.x / 8) in
+// [slice0[b], slice0[b + 1]) work on bin b, slice0[b + 1] - slice0[b] =
+// ceil(units of bin b / 8) of them per XCD.
+struct NumMultiBins {
+    int32_t n;
+    int32_t variant[kMaxBins];
+    int32_t off[kMaxBins];
+    int32_t count[kMaxBins];
+    int32_t slice0[kMaxBins + 1];
+};
 
 constexpr int kGlobalSortChunk = 4096;  // LDS chunk of the global-bin column sort
 
@@ -82,6 +128,14 @@ constexpr int max_num_lds() {
     for (int i = 0; i < kNumLdsVariants; ++i) {
         const int b = 4 * kNumVariants[i].r * num_row_ints(kNumVariants[i]);
         m = m > b ? m : b;
+    }
+    return m;
+}
+constexpr int max_num_multi_lds(int set) {
+    int m = 0;
+    for (int v = 0; v < kNumLdsVariants; ++v) {
+        const int b = 4 * kNumVariants[v].r * num_row_ints(kNumVariants[v]);
+        if (num_multi_set(v) == set) m = m > b ? m : b;
     }
     return m;
 }
@@ -130,6 +184,11 @@ void launch_sym_global(const ProbSet<V>& ps, const int2* fail_items, const long 
 template <typename V>
 hipError_t launch_symnum_bin(int variant, const ProbSet<V>& ps, const RowItem* items, int item_off, int count,
                              DevMeta* meta, int grid, hipStream_t s);
+// fused pass: one launch over the units of several bins whose variants are
+// in merged set `set` (a workgroup per unit)
+template <typename V>
+hipError_t launch_symnum_multi(int set, const ProbSet<V>& ps, const RowItem* items, const NumMultiBins& mb,
+                               DevMeta* meta, hipStream_t s);
 template <typename V>
 void launch_symnum_global(const ProbSet<V>& ps, const RowItem* items, const long long* gpool_off, int item_off,
                           int count, int* pool, DevMeta* meta, hipStream_t s);

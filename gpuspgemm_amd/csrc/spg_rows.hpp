@@ -95,14 +95,6 @@ struct alignas(8) ChunkMeta {
     V av[VALS ? EC : 1];     // a_ik
 };
 
-// Row of B selected by entry e of A, as the NIP pass recorded it.
-template <typename V>
-__device__ __forceinline__ void entry_row(const GProb<V>& P, int e, int& bs, int& len) {
-    const long long r = P.bref[e];
-    bs = bref_start(r);
-    len = bref_len(r);
-}
-
 // Load entries [ebase, min(ebase + EC, a1)) of A's row into `cm`; returns
 // the chunk's product count.  Each lane loads EC/L consecutive entries.  A
 // group of one whole wave keeps only entries with a non-empty B row, in
@@ -115,17 +107,26 @@ __device__ __forceinline__ int load_chunk(const GProb<V>& P, int ebase, int a1, 
     constexpr int PER = EC / L;
     int len[PER], bs[PER];
     V av[PER];
-    int local = 0, filled = 0;
+    // all loads first, into registers with no default to merge with (a
+    // merge at the end of the branch would wait for each load in turn)
+    long long ref[PER];
+    V a[PER];
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
         const int e = ebase + lr * PER + t;
-        len[t] = 0;
-        bs[t] = 0;
-        av[t] = V(0);
         if (e < a1) {
-            if constexpr (VALS) av[t] = P.a_val[e];
-            entry_row<V>(P, e, bs[t], len[t]);
+            ref[t] = P.bref[e];
+            if constexpr (VALS) a[t] = P.a_val[e];
         }
+    }
+    int local = 0, filled = 0;
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+        const bool in = ebase + lr * PER + t < a1;
+        bs[t] = in ? bref_start(ref[t]) : 0;
+        len[t] = in ? bref_len(ref[t]) : 0;
+        av[t] = V(0);
+        if constexpr (VALS) av[t] = in ? a[t] : V(0);
         local += len[t];
         filled += len[t] > 0 ? 1 : 0;
     }
@@ -160,24 +161,35 @@ __device__ __forceinline__ int load_chunk(const GProb<V>& P, int ebase, int a1, 
     return total;
 }
 
-// Entry of product p (0 <= p < total): the largest e with off[e] <= p.
-template <int EC>
-__device__ __forceinline__ int chunk_find(const int* off, int p) {
-    int e = 0;
+// Entry of product p[j] (0 <= p[j] < total) for every slot j: the largest
+// e with off[e] <= p[j].  The slots' searches advance in lockstep, so each
+// step issues PMAX independent LDS reads and waits once.
+template <int EC, int PMAX>
+__device__ __forceinline__ void chunk_find_all(const int* off, const int (&p)[PMAX], int (&e)[PMAX]) {
 #pragma unroll
-    for (int step = EC / 2; step > 0; step >>= 1)
-        if (off[e + step] <= p) e += step;
-    return e;
+    for (int j = 0; j < PMAX; ++j) e[j] = 0;
+#pragma unroll
+    for (int step = EC / 2; step > 0; step >>= 1) {
+        int o[PMAX];
+#pragma unroll
+        for (int j = 0; j < PMAX; ++j) o[j] = off[e[j] + step];
+#pragma unroll
+        for (int j = 0; j < PMAX; ++j) e[j] += o[j] <= p[j] ? step : 0;
+    }
 }
 
 // Fetch up to PMAX products of a chunk: slot j holds product
 // base + j*L + lr.  ok[j] tells which slots are real.  `prior` counts the
 // entries that start before `base` (0 for a chunk's first round) and is
-// advanced past this round.
+// advanced past this round.  Three phases, so that a lane's gathers are in
+// flight together: the entry of every slot, then the loads of B's columns
+// and values for all slots, then the products a_ik * b_kj (a multiply next
+// to its load would wait for that load before the next one is issued).
 template <typename V, int L, int EC, int PMAX, bool VALS>
 __device__ __forceinline__ void fetch_products(const GProb<V>& P, ChunkMeta<V, EC, VALS>* cm, int total, int base,
                                                int lr, const int (&start)[EC / L], int& prior, int (&key)[PMAX],
                                                V (&prod)[PMAX], bool (&ok)[PMAX]) {
+    int e[PMAX];
     if constexpr (L == kWave) {
         // slot j covers bitmap word j; lane lr counts the set bits at or
         // below its own position.  (Moving the word to scalar registers and
@@ -193,39 +205,50 @@ __device__ __forceinline__ void fetch_products(const GProb<V>& P, ChunkMeta<V, E
             if (start[t] >= 0 && q >= 0 && q < L * PMAX) atomicOr(&bits[q >> 6], 1ull << (q & 63));
         }
         wave_sync();
+        unsigned long long w[PMAX];
+#pragma unroll
+        for (int j = 0; j < PMAX; ++j) w[j] = bits[j];
         const unsigned long long upto = ~0ull >> (63 - lr);  // positions 0..lr
         int cnt = prior;  // entries starting before slot j's window
 #pragma unroll
         for (int j = 0; j < PMAX; ++j) {
-            const unsigned long long w = bits[j];
-            const int p = base + j * L + lr;
-            ok[j] = p < total;
-            key[j] = 0;
-            prod[j] = V(0);
-            if (ok[j]) {
-                const int e = cnt + __popcll(w & upto) - 1;
-                const int at = p + cm->d[e];
-                key[j] = P.b_col[at];
-                if constexpr (VALS) prod[j] = cm->av[e] * P.b_val[at];
-            }
-            cnt += __popcll(w);
+            e[j] = max(cnt + __popcll(w[j] & upto) - 1, 0);  // clamped for slots past the chunk
+            cnt += __popcll(w[j]);
         }
         prior = cnt;
         wave_sync();  // the bitmap is rewritten by the next round
     } else {
+        int p[PMAX];
 #pragma unroll
-        for (int j = 0; j < PMAX; ++j) {
-            const int p = base + j * L + lr;
-            ok[j] = p < total;
-            key[j] = 0;
-            prod[j] = V(0);
-            if (ok[j]) {
-                const int e = chunk_find<EC>(cm->off, p);
-                const int at = cm->d[e] + p - cm->off[e];
-                key[j] = P.b_col[at];
-                if constexpr (VALS) prod[j] = cm->av[e] * P.b_val[at];
-            }
+        for (int j = 0; j < PMAX; ++j) p[j] = base + j * L + lr;
+        chunk_find_all<EC, PMAX>(cm->off, p, e);
+    }
+    int at[PMAX];
+#pragma unroll
+    for (int j = 0; j < PMAX; ++j) {
+        const int p = base + j * L + lr;
+        ok[j] = p < total;
+        // L == 64: d holds the entry's B position minus its product offset
+        at[j] = L == kWave ? p + cm->d[e[j]] : cm->d[e[j]] + p - cm->off[e[j]];
+    }
+    // The loads write registers that hold nothing else (no default to
+    // merge with at the end of the branch, which would cost a copy and a
+    // wait for the load inside the loop); defaults are selected afterwards.
+    int kk[PMAX];
+    V bv[PMAX];
+#pragma unroll
+    for (int j = 0; j < PMAX; ++j)
+        if (ok[j]) {
+            kk[j] = P.b_col[at[j]];
+            if constexpr (VALS) bv[j] = P.b_val[at[j]];
         }
+#pragma unroll
+    for (int j = 0; j < PMAX; ++j) {
+        key[j] = ok[j] ? kk[j] : 0;
+        if constexpr (VALS)
+            prod[j] = ok[j] ? bv[j] * cm->av[e[j]] : V(0);
+        else
+            prod[j] = V(0);
     }
 }
 
@@ -234,16 +257,20 @@ template <typename V, int EC, bool VALS>
 __device__ __forceinline__ int load_chunk_lane(const GProb<V>& P, int ebase, int a1, ChunkMeta<V, EC, VALS>& cm) {
     int run = 0;
     int bs[EC], len[EC];
+    long long ref[EC];
+    V a[EC];
+#pragma unroll
+    for (int t = 0; t < EC; ++t)
+        if (ebase + t < a1) {
+            ref[t] = P.bref[ebase + t];
+            if constexpr (VALS) a[t] = P.a_val[ebase + t];
+        }
 #pragma unroll
     for (int t = 0; t < EC; ++t) {
-        const int e = ebase + t;
-        bs[t] = 0;
-        len[t] = 0;
-        if constexpr (VALS) cm.av[t] = V(0);
-        if (e < a1) {
-            if constexpr (VALS) cm.av[t] = P.a_val[e];
-            entry_row<V>(P, e, bs[t], len[t]);
-        }
+        const bool in = ebase + t < a1;
+        bs[t] = in ? bref_start(ref[t]) : 0;
+        len[t] = in ? bref_len(ref[t]) : 0;
+        if constexpr (VALS) cm.av[t] = in ? a[t] : V(0);
     }
 #pragma unroll
     for (int t = 0; t < EC; ++t) {
@@ -254,29 +281,44 @@ __device__ __forceinline__ int load_chunk_lane(const GProb<V>& P, int ebase, int
     return run;
 }
 
+// The same for a lane that owns its row (metadata in registers): every
+// slot's B position first, then all loads, then the products.
 template <typename V, int EC, int PMAX, bool VALS>
 __device__ __forceinline__ void fetch_products_lane(const GProb<V>& P, const ChunkMeta<V, EC, VALS>& cm, int total,
                                                     int base, int (&key)[PMAX], V (&prod)[PMAX], bool (&ok)[PMAX]) {
+    int q[PMAX];
 #pragma unroll
     for (int j = 0; j < PMAX; ++j) {
         const int p = base + j;
         ok[j] = p < total;
-        key[j] = 0;
-        prod[j] = V(0);
-        if (ok[j]) {
-            int off = cm.off[0], bs = cm.d[0];
-            V av = VALS ? cm.av[0] : V(0);
+        int off = cm.off[0], bs = cm.d[0];
 #pragma unroll
-            for (int t = 1; t < EC; ++t) {
-                if (cm.off[t] <= p) {
-                    off = cm.off[t];
-                    bs = cm.d[t];
-                    if constexpr (VALS) av = cm.av[t];
-                }
+        for (int t = 1; t < EC; ++t)
+            if (cm.off[t] <= p) {
+                off = cm.off[t];
+                bs = cm.d[t];
             }
-            const int q = bs + p - off;
-            key[j] = P.b_col[q];
-            if constexpr (VALS) prod[j] = av * P.b_val[q];
+        q[j] = bs + p - off;
+    }
+    int kk[PMAX];
+    V bv[PMAX];
+#pragma unroll
+    for (int j = 0; j < PMAX; ++j)
+        if (ok[j]) {
+            kk[j] = P.b_col[q[j]];
+            if constexpr (VALS) bv[j] = P.b_val[q[j]];
+        }
+#pragma unroll
+    for (int j = 0; j < PMAX; ++j) {
+        key[j] = ok[j] ? kk[j] : 0;
+        prod[j] = V(0);
+        if constexpr (VALS) {
+            const int p = base + j;
+            V av = cm.av[0];
+#pragma unroll
+            for (int t = 1; t < EC; ++t)
+                if (cm.off[t] <= p) av = cm.av[t];
+            prod[j] = ok[j] ? bv[j] * av : V(0);
         }
     }
 }

@@ -33,6 +33,7 @@
 #define __restrict__
 #define __align__(n) alignas(n)
 #define __launch_bounds__(...)
+#define SPG_WAVES_PER_EU(n)  // register budgets mean nothing to host threads
 
 struct dim3 {
     unsigned x = 1, y = 1, z = 1;

@@ -109,6 +109,20 @@ def test_graph_replay(emu, tmp_path):
     assert out.count("structure identical") == 9
 
 
+def test_merged_launches(emu, tmp_path):
+    """A fused table with a bin for every variant of the merged small-row
+    launch and four bins of the merged longer-row launch: the rows of each
+    set's bins share one launch, a workgroup per unit; every result must
+    equal the host product."""
+    assert sorted(v["id"] for v in _NV if v["merged"] == 0) == [10, 11, 12, 13, 14, 16]
+    assert sorted(v["id"] for v in _NV if v["merged"] == 1) == [4, 5, 6, 7, 8]
+    table = ("10 4 4\n14 8 8\n11 12 16\n16 16 16\n12 32 32\n13 64 64\n"
+             f"4 96 96\n5 160 128\n6 512 256\n7 {INT_MAX} 1024\n{NUM_GLOBAL} {INT_MAX} {INT_MAX}\n")
+    A = gen.powerlaw(m=2500, seed=21)
+    run_emu(emu, tmp_path, A, A, tables={1: table})
+    run_emu(emu, tmp_path, A.astype(np.float32), A.astype(np.float32), "--f32", tables={1: table})
+
+
 def test_rectangular_and_empty_rows(emu, tmp_path):
     A = gen.random_csr(120, 90, 0.05, seed=1, empty_rows=0.3)
     B = gen.random_csr(90, 70, 0.06, seed=2, empty_rows=0.2)
