@@ -159,6 +159,7 @@ def tune(workloads: list[Workload], reps: int = 5, passes: int = 2, log=print) -
                 cands = [(v, slot[1], slot[2]) for v in (sym_candidates if phase == 0 else num_candidates)(slot, vs)
                          if v != slot[0]]
                 cands += cap_moves(phase, slot, vs)
+                log(f"phase {phase} slot {i} {slot}: {len(cands)} candidates, score {best:.4f}")
                 for cand in cands:
                     trial = {p: list(t) for p, t in tables.items()}
                     trial[phase][i] = cand

@@ -4,6 +4,7 @@
 // finished the kernel no longer hold others back.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
@@ -329,7 +330,25 @@ void block_barrier() {
     g_wave_bar[tl_thread.x / 64].park(1);
     g_block_bar.arrive(unpark_all);
 }
-void wave_barrier() { g_wave_bar[tl_thread.x / 64].arrive(); }
+// EMU_SLOW_WAVE=w:us makes wave w of workgroup 0 of every launch sleep us
+// microseconds at each wave barrier, so that the other waves of the
+// workgroup run ahead of it: code that lets a wave reuse LDS another wave
+// may still be using without a workgroup barrier then fails.
+struct SlowWave {
+    int wave = -1;
+    int us = 0;
+};
+static SlowWave slow_wave() {
+    SlowWave s;
+    if (const char* e = std::getenv("EMU_SLOW_WAVE")) std::sscanf(e, "%d:%d", &s.wave, &s.us);
+    return s;
+}
+void wave_barrier() {
+    static const SlowWave slow = slow_wave();
+    if (slow.wave >= 0 && tl_block.x == 0 && static_cast<int>(tl_thread.x / 64) == slow.wave)
+        std::this_thread::sleep_for(std::chrono::microseconds(slow.us));
+    g_wave_bar[tl_thread.x / 64].arrive();
+}
 void exchange_write(std::uint64_t v) { g_slots[tl_thread.x] = v; }
 std::uint64_t exchange_read(int l) { return g_slots[(tl_thread.x / 64) * 64 + static_cast<unsigned>(l)]; }
 

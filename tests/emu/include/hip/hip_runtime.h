@@ -31,7 +31,7 @@
 #define SPG_GLOBAL  // host threads have one address space
 #define __noinline__
 #define __restrict__
-#define __align__(n) alignas(n)
+#define __align__(n) __attribute__((aligned(n)))
 #define __launch_bounds__(...)
 #define SPG_WAVES_PER_EU(n)  // register budgets mean nothing to host threads
 

@@ -35,16 +35,21 @@ NUM = {v["id"]: (v["lanes"], v["chunk"], v["capacity"]) for v in _NV if not v["g
 NUM_GLOBAL = next(v["id"] for v in _NV if v["global"])
 
 
+def build_emu(csrc: str, exe: str) -> None:
+    """The emulation build of the kernel and host sources in `csrc`."""
+    srcs = [os.path.join(csrc, f) for f in ("k_rows.hip", "k_sym.hip", "k_symnum.hip", "api.hip")]
+    cmd = ["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer", "-pthread",
+           "-Wno-unused-result", "-I", os.path.join(EMU, "include"), "-I", os.path.join(ROOT, "include"),
+           "-I", csrc, "-I", os.path.join(ROOT, "gpuspgemm_amd", "apps"), "-x", "c++", *srcs, "-x", "none",
+           os.path.join(EMU, "emu_runtime.cpp"), os.path.join(EMU, "emu_main.cpp"), "-o", exe]
+    subprocess.run(cmd, check=True)
+
+
 @pytest.fixture(scope="module")
 def emu():
     os.makedirs(BUILD, exist_ok=True)
     tmp = f".{os.getpid()}"
-    srcs = [os.path.join(CSRC, f) for f in ("k_rows.hip", "k_sym.hip", "k_symnum.hip", "api.hip")]
-    cmd = ["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer", "-pthread",
-           "-Wno-unused-result", "-I", os.path.join(EMU, "include"), "-I", os.path.join(ROOT, "include"),
-           "-I", os.path.join(ROOT, "gpuspgemm_amd", "apps"), "-x", "c++", *srcs, "-x", "none",
-           os.path.join(EMU, "emu_runtime.cpp"), os.path.join(EMU, "emu_main.cpp"), "-o", EXE + tmp]
-    subprocess.run(cmd, check=True)
+    build_emu(CSRC, EXE + tmp)
     os.replace(EXE + tmp, EXE)  # parallel test workers may be running the previous build
     return EXE
 
@@ -121,6 +126,52 @@ def test_merged_launches(emu, tmp_path):
     A = gen.powerlaw(m=2500, seed=21)
     run_emu(emu, tmp_path, A, A, tables={1: table})
     run_emu(emu, tmp_path, A.astype(np.float32), A.astype(np.float32), "--f32", tables={1: table})
+
+
+def _alternating_units_case():
+    """Rows whose symbolic units alternate LDS layouts inside one workgroup
+    of the emulated grid (8 workgroups, one XCD slice each): bins of the
+    larger-table launch with variants 6, 5, 6 (as in the tuned defaults)
+    holding 1, 1 and 22 units of 4 rows, so that workgroup 0 runs a unit of
+    each in turn.  B has 16 entries per row; a row of A with k entries has
+    NIP 16k."""
+    nb = 400
+    brows = [np.unique(gen.rand_below(31, 1 + r, np.arange(16), np.full(16, 3000))) for r in range(nb)]
+    brows = [np.concatenate([x, np.arange(3000, 3000 + 16 - x.size)]) for x in brows]  # exactly 16 each
+    brpt = np.concatenate([[0], np.cumsum([x.size for x in brows])]).astype(np.int32)
+    bcol = np.concatenate(brows).astype(np.int32)
+    B = HostCSR(nb, 3100, brpt, bcol, gen.rand_unit(32, 1, np.arange(bcol.size)) * 2 - 1)
+    lengths = [6] * 4 + [12] * 4 + [24] * 88  # NIP 96 (bin of v6), 192 (v5), 384 (v6)
+    arows = [np.unique(gen.rand_below(33, 1 + i, np.arange(k), np.full(k, nb)))[:k] for i, k in enumerate(lengths)]
+    arows = [np.concatenate([x, np.setdiff1d(np.arange(nb), x)[:k - x.size]]) for x, k in zip(arows, lengths)]
+    arows = [np.sort(x) for x in arows]
+    rpt = np.concatenate([[0], np.cumsum([x.size for x in arows])]).astype(np.int32)
+    col = np.concatenate(arows).astype(np.int32)
+    A = HostCSR(len(arows), nb, rpt, col, gen.rand_unit(34, 1, np.arange(col.size)) * 2 - 1)
+    table = "".join(f"{v} {cap} {INT_MAX}\n" for v, cap in
+                    [(0, 8), (1, 16), (2, 32), (3, 64), (6, 128), (5, 256), (6, 768), (7, 2048), (8, 4096),
+                     (SYM_MAX, INT_MAX)])
+    return A, B, table
+
+
+def test_symbolic_units_with_a_wave_held_back(emu, tmp_path):
+    """Symbolic units of alternating LDS layouts (variants 6, 5, 6) in one
+    workgroup while wave 1 of that workgroup is held back at every wave
+    barrier, so the other waves run a unit or two ahead of it: the
+    workgroup barrier between units of different layouts must keep them
+    off the slots wave 1 still uses, and the result must equal the host
+    product.  (Built without that barrier, the same run gives a wrong count
+    or an out-of-bounds read only in some runs, so it is not kept as a
+    planted-mistake check.)"""
+    A, B, table = _alternating_units_case()
+    fa, fb, ft = tmp_path / "a.csr", tmp_path / "b.csr", tmp_path / "sym.txt"
+    A.save_to_filename(str(fa))
+    B.save_to_filename(str(fb))
+    ft.write_text(table)
+    res = subprocess.run([emu, str(fa), str(fb), "--sym-table", str(ft)], stdout=subprocess.PIPE,
+                         stderr=subprocess.STDOUT, text=True, timeout=600,
+                         env=dict(os.environ, EMU_SLOW_WAVE="1:300"))
+    assert res.returncode == 0 and "structure identical" in res.stdout, res.stdout[-3000:]
 
 
 def test_rectangular_and_empty_rows(emu, tmp_path):
