@@ -288,7 +288,33 @@ def _served_request(k: int) -> dict:
     return gen.served_request(k)
 
 
-def run_served(n_req: int, load: float, budget_gb: float, max_batch: int) -> dict:
+def served_bound(dev, nips, iso, rows) -> dict:
+    """What any schedule of these requests can gain over the serial plain
+    path, measured: every request known at once and run in one batched pass
+    per value type (best of 3 wall-clock runs), and the share of the serial
+    time taken by the requests of at least 2^17 rows, which keep the GPU
+    busy on their own."""
+    import gpuspgemm_amd as g
+    groups = {}
+    for i, (a, b) in enumerate(dev):
+        groups.setdefault(str(a.dtype), []).append(i)
+    best = None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        for idx in groups.values():
+            for c in g.proposal_batched([dev[i][0] for i in idx], [dev[i][1] for i in idx]):
+                c.free()
+        t = time.perf_counter() - t0
+        best = t if best is None else min(best, t)
+    total, serial = float(sum(nips)), float(np.sum(iso))
+    big = np.array(rows) >= (1 << 17)
+    return {"batched_all_s": best, "batched_all_products_per_s": total / best,
+            "batched_all_vs_serial": serial / best,
+            "serial_share_rows_ge_2^17": float(np.sum(iso[big]) / serial),
+            "requests_rows_ge_2^17": int(big.sum())}
+
+
+def run_served(n_req: int, load: float, budget_gb: float, max_batch: int, bound: bool = False) -> dict:
     """Served form: Poisson arrivals of mixed requests (70 % FP64, 30 % FP32;
     generator drawn from uniform8 / power-law / stencil / batched instance;
     rows log-uniform in [2^10, 2^20]) at an offered load of `load` times the
@@ -338,13 +364,14 @@ def run_served(n_req: int, load: float, budget_gb: float, max_batch: int) -> dic
         th.join()
         makespan = time.perf_counter() - start
         st = srv.stats()
-    for a, b in dev:
-        a.free()
-        b.free()
     lat = np.array(lat)
     iso = np.array(iso)
     total = float(sum(nips))
-    return {"config": {"workload": "served", "requests": n_req, "rows": "log-uniform [2^10, 2^20]",
+    extra = {"bound": served_bound(dev, nips, iso, [a.m for a, _ in dev])} if bound else {}
+    for a, b in dev:
+        a.free()
+        b.free()
+    return {**extra, "config": {"workload": "served", "requests": n_req, "rows": "log-uniform [2^10, 2^20]",
                        "mix": "70% f64 / 30% f32; uniform8, power-law, stencil, batched instance", "seed": 4,
                        "offered_load_req_per_s": round(rate, 2), "load_vs_serial": load,
                        "budget_gb": budget_gb, "max_batch": max_batch},
@@ -456,7 +483,7 @@ def main(argv=None) -> int:
         }
     if args.form == "all":
         line["batched"] = run_batched(args.instances, 3)
-        line["served"] = run_served(args.requests, args.load, args.budget_gb, args.max_batch)
+        line["served"] = run_served(args.requests, args.load, args.budget_gb, args.max_batch, bound=True)
         line["served_moderate"] = run_served(args.requests, args.mod_load, args.budget_gb, args.max_batch)
         line["served_saturated"] = run_served(args.requests, args.sat_load, args.budget_gb, args.max_batch)
     if not args.no_cpu:

@@ -1,17 +1,18 @@
 """Per-multiply GPU timeline from a rocprofv3 kernel trace.
 
-    python -m gpuspgemm_amd.timeline <kernel_trace.csv> [--last N]
+    python -m gpuspgemm_amd.timeline <kernel_trace.csv> [--last N] [--skip K]
 
 The trace (`rocprofv3 --kernel-trace --output-format csv`) lists every
 dispatch with its hardware queue and start / end timestamps.  A multiply
 starts with its `k_nip` launch (or the `k_clear` right before it); for each
-of the last N multiplies this prints the kernels in start order with their
+of the last N multiplies before the final K this prints the kernels in start order with their
 queue, the idle gaps (no kernel of the process running), the busy fraction
 (union of kernel intervals over the span from the first start to the last
 end) and the time of the two phases: symbolic (first kernel to the end of
 the read-back that carries nnz(C)) and fused (first fused launch to the end
-of the last read-back).  The profiler adds its own per-dispatch cost, so
-spans here are longer than the unprofiled wall clock.
+of the last read-back), then the means of these over the N multiplies.
+The profiler adds its own per-dispatch cost, so spans here are longer than
+the unprofiled wall clock.
 """
 from __future__ import annotations
 
@@ -26,6 +27,9 @@ def short_name(name: str) -> str:
     m = re.search(r"k_symnum_bin<(?:double|float), (\d+)>", n)
     if m:
         return "symnum_v" + m.group(1)
+    m = re.search(r"k_symnum_multi<(?:double|float), (\d+)>", n)
+    if m:
+        return "symnum_multi" + m.group(1)
     if "k_sym_multi" in n:
         return "sym_multi_small" if ", 0, 4>" in n else "sym_multi"
     m = re.match(r"(?:void )?(k_\w+)", n)
@@ -82,10 +86,14 @@ def summarize(step: list[dict]) -> dict:
             "queues": {q: (v[0] / 1e3, v[1]) for q, v in sorted(queues.items())}}
 
 
-def report(path: str, last: int) -> str:
-    steps = split_multiplies(load(path))[-last:]
+def report(path: str, last: int, skip: int = 0) -> str:
+    steps = split_multiplies(load(path))
+    steps = steps[max(0, len(steps) - skip - last):len(steps) - skip]
     lines = []
+    means = {k: 0.0 for k in ("span_us", "busy_us", "busy_frac", "symbolic_us", "readback_to_fused_us", "fused_us")}
     for n, st in enumerate(steps):
+        for k in means:
+            means[k] += summarize(st)[k] / len(steps)
         s = summarize(st)
         lines.append(f"multiply {n + 1} of the last {len(steps)}: span {s['span_us']:.1f} us, busy "
                      f"{s['busy_us']:.1f} us ({100 * s['busy_frac']:.0f} %), symbolic phase {s['symbolic_us']:.1f} us, "
@@ -100,6 +108,10 @@ def report(path: str, last: int) -> str:
         idle = [g for g in s["gaps"] if g[1] >= 1.0]
         lines.append("  idle gaps >= 1 us (at, length, before): " + (
             ", ".join(f"{a:.1f}+{l:.1f} before {b}" for a, l, b in idle) if idle else "none"))
+    if steps:
+        lines.append(f"mean of {len(steps)}: span {means['span_us']:.1f} us, busy {means['busy_us']:.1f} us "
+                     f"({100 * means['busy_frac']:.0f} %), symbolic phase {means['symbolic_us']:.1f} us, read-back -> "
+                     f"first fused launch {means['readback_to_fused_us']:.1f} us, fused pass {means['fused_us']:.1f} us")
     return "\n".join(lines)
 
 
@@ -107,8 +119,9 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("trace")
     ap.add_argument("--last", type=int, default=3)
+    ap.add_argument("--skip", type=int, default=0, help="leave out the final K multiplies (e.g. profiled steps)")
     args = ap.parse_args(argv)
-    print(report(args.trace, args.last))
+    print(report(args.trace, args.last, args.skip))
     return 0
 
 
