@@ -444,7 +444,7 @@ def main(argv=None) -> int:
         print(json.dumps(run_batched(args.instances, 3)), flush=True)
         return 0
     if args.form == "served":
-        print(json.dumps(run_served(args.requests, args.load, args.budget_gb, args.max_batch)), flush=True)
+        print(json.dumps(run_served(args.requests, args.load, args.budget_gb, args.max_batch, bound=True)), flush=True)
         return 0
 
     A, cfg = make_workload(args.workload)
