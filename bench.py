@@ -308,10 +308,21 @@ def served_bound(dev, nips, iso, rows) -> dict:
         best = t if best is None else min(best, t)
     total, serial = float(sum(nips)), float(np.sum(iso))
     big = np.array(rows) >= (1 << 17)
+    # Queueing at the offered load of the serial rate even with the best
+    # measured speedup (service times scaled by it): utilisation rho and
+    # the Pollaczek-Khinchine mean wait of an M/G/1 queue,
+    # rho / (1 - rho) * E[S^2] / (2 E[S]).
+    speed = serial / best
+    svc = iso / speed
+    rho = 1.0 / speed
+    mean_wait = rho / (1.0 - rho) * float(np.mean(svc ** 2)) / (2.0 * float(np.mean(svc))) if rho < 1 else None
     return {"batched_all_s": best, "batched_all_products_per_s": total / best,
-            "batched_all_vs_serial": serial / best,
+            "batched_all_vs_serial": speed,
             "serial_share_rows_ge_2^17": float(np.sum(iso[big]) / serial),
-            "requests_rows_ge_2^17": int(big.sum())}
+            "requests_rows_ge_2^17": int(big.sum()),
+            "mg1_rho_at_serial_rate": rho,
+            "mg1_mean_wait_ms_at_serial_rate": mean_wait * 1e3 if mean_wait is not None else None,
+            "isolated_p99_ms": float(np.percentile(iso, 99) * 1e3)}
 
 
 def run_served(n_req: int, load: float, budget_gb: float, max_batch: int, bound: bool = False) -> dict:

@@ -212,8 +212,11 @@ __device__ __forceinline__ void fetch_products(const GProb<V>& P, ChunkMeta<V, E
         int cnt = prior;  // entries starting before slot j's window
 #pragma unroll
         for (int j = 0; j < PMAX; ++j) {
-            e[j] = max(cnt + __popcll(w[j] & upto) - 1, 0);  // clamped for slots past the chunk
-            cnt += __popcll(w[j]);
+            // clamped for slots past the chunk (int operands: a mixed
+            // int / unsigned max resolves to the double overload)
+            const int before = cnt + static_cast<int>(__popcll(w[j] & upto)) - 1;
+            e[j] = before > 0 ? before : 0;
+            cnt += static_cast<int>(__popcll(w[j]));
         }
         prior = cnt;
         wave_sync();  // the bitmap is rewritten by the next round
