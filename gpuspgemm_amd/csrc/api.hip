@@ -448,7 +448,7 @@ struct Context {
     std::mutex mu;  // tables, the lane map, profiling records
     BinTable sym{};
     BinTable num{};
-    int occ_sym_multi[2] = {1, 1}, occ_sym_max = 1;
+    int occ_sym_multi[kSymParts] = {1, 1, 1}, occ_sym_max = 1;
     int occ_num[2][kNumLdsVariants] = {};
     Lane lane;  // graph replay and profiling run here, one call at a time
     Lane noalloc_lane;             // spg_multiply_noalloc, one call at a time
@@ -578,8 +578,7 @@ spg_status ensure_context(Context*& out) {
     default_tables(c.sym, c.num);
     SPG_TRY(check_sym_table(c.sym));  // a stale tuned_tables.inc fails here, loudly
     SPG_TRY(check_num_table(c.num));
-    c.occ_sym_multi[0] = occupancy_sym_multi<double>(0);
-    c.occ_sym_multi[1] = occupancy_sym_multi<double>(1);
+    for (int part = 0; part < kSymParts; ++part) c.occ_sym_multi[part] = occupancy_sym_multi<double>(part);
     c.occ_sym_max = occupancy_sym_max<double>();
     for (int v = 0; v < kNumLdsVariants; ++v) {
         c.occ_num[0][v] = occupancy_symnum_bin<float>(v);
@@ -892,10 +891,12 @@ struct Job : JobBase {
         // The largest-table bin runs on an aux stream beside the multi-bin
         // launches (queued first: its workgroups take the CUs they need
         // before the multi-bin grids fill them, and those without a row
-        // exit at once).  With both multi-bin parts in use the larger-table
-        // part runs on the second aux stream: its workgroups take the CUs
-        // that the small-table grid leaves as its last units drain.  The
-        // three launches count disjoint rows.
+        // exit at once); the part of the two largest multi-bin tables
+        // (rows of a workgroup each, few of them) follows it on that
+        // stream.  With the small-table part in use the mid-table part runs
+        // on the second aux stream: its workgroups take the CUs that the
+        // small-table grid leaves as its last units drain.  The launches
+        // count disjoint rows.
         SPG_HIP(hipEventRecord(L.ev_ready, s));
         SPG_HIP(hipStreamWaitEvent(L.aux[0], L.ev_ready, 0));
         {
@@ -906,16 +907,17 @@ struct Job : JobBase {
             prof.end();
         }
         const int n_multi = sym.n - 1;
-        bool part_used[2] = {false, false};
-        for (int b = 0; b < n_multi; ++b) part_used[sym.variant[b] < kSymSmallVariants ? 0 : 1] = true;
+        bool part_used[kSymParts] = {};
+        for (int b = 0; b < n_multi; ++b) part_used[sym_part(sym.variant[b])] = true;
         const bool split = part_used[0] && part_used[1];
         if (split) SPG_HIP(hipStreamWaitEvent(L.aux[1], L.ev_ready, 0));
-        for (int part = 0; part < 2; ++part) {
+        static const char* const kPartName[kSymParts] = {"sym_multi_small", "sym_multi", "sym_multi_large"};
+        for (int part : {2, 1, 0}) {
             if (!part_used[part]) continue;
-            const hipStream_t where = split && part == 1 ? L.aux[1] : s;
+            const hipStream_t where = part == 2 ? L.aux[0] : (split && part == 1 ? L.aux[1] : s);
             int grid = c.cus * c.occ_sym_multi[part];
             grid = (grid + kXcds - 1) / kXcds * kXcds;
-            prof.begin(part == 0 ? "sym_multi_small" : "sym_multi", total_rows, grid, where);
+            prof.begin(kPartName[part], total_rows, grid, where);
             launch_sym_multi<V>(part, ps, items1, static_cast<int>(total_rows), sym, n_multi, meta, grid, where);
             prof.end();
         }

@@ -75,7 +75,7 @@ constexpr int sym_multi_lds() {
 // a kernel takes the register budget of its largest variant, and theirs is
 // far smaller, so twice as many of their waves fit on a CU.
 template <typename V, int LO, int HI>
-__global__ __launch_bounds__(kMultiThreads) SPG_WAVES_PER_EU(kSymMultiMinWaves[LO == 0 ? 0 : 1]) void k_sym_multi(ProbSet<V> ps, const int2* __restrict__ items,
+__global__ __launch_bounds__(kMultiThreads) SPG_WAVES_PER_EU(kSymMultiMinWaves[sym_part(LO)]) void k_sym_multi(ProbSet<V> ps, const int2* __restrict__ items,
                                                              int bin_cap, BinTable sym, int n_multi,
                                                              const DevMeta* __restrict__ meta) {
     static_assert(sym_multi_lds<LO, HI>() <= kSymMultiLds, "symbolic multi-bin LDS budget");
@@ -242,11 +242,14 @@ template <typename V>
 void launch_sym_multi(int part, const ProbSet<V>& ps, const int2* items, int bin_cap, const BinTable& sym,
                       int n_multi, const DevMeta* meta, int grid, hipStream_t s) {
     if (part == 0)
-        hipLaunchKernelGGL((k_sym_multi<V, 0, kSymSmallVariants>), dim3(grid), dim3(kMultiThreads), 0, s, ps, items,
-                           bin_cap, sym, n_multi, meta);
+        hipLaunchKernelGGL((k_sym_multi<V, kSymPartLo[0], kSymPartHi[0]>), dim3(grid), dim3(kMultiThreads), 0, s, ps,
+                           items, bin_cap, sym, n_multi, meta);
+    else if (part == 1)
+        hipLaunchKernelGGL((k_sym_multi<V, kSymPartLo[1], kSymPartHi[1]>), dim3(grid), dim3(kMultiThreads), 0, s, ps,
+                           items, bin_cap, sym, n_multi, meta);
     else
-        hipLaunchKernelGGL((k_sym_multi<V, kSymSmallVariants, kSymMultiVariants>), dim3(grid), dim3(kMultiThreads), 0,
-                           s, ps, items, bin_cap, sym, n_multi, meta);
+        hipLaunchKernelGGL((k_sym_multi<V, kSymPartLo[2], kSymPartHi[2]>), dim3(grid), dim3(kMultiThreads), 0, s, ps,
+                           items, bin_cap, sym, n_multi, meta);
 }
 
 template <typename V>
@@ -267,10 +270,12 @@ template <typename V>
 int occupancy_sym_multi(int part) {
     int n = 0;
     const hipError_t e =
-        part == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sym_multi<V, 0, kSymSmallVariants>,
-                                                                 kMultiThreads, 0)
-                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &n, k_sym_multi<V, kSymSmallVariants, kSymMultiVariants>, kMultiThreads, 0);
+        part == 0   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          &n, k_sym_multi<V, kSymPartLo[0], kSymPartHi[0]>, kMultiThreads, 0)
+        : part == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          &n, k_sym_multi<V, kSymPartLo[1], kSymPartHi[1]>, kMultiThreads, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          &n, k_sym_multi<V, kSymPartLo[2], kSymPartHi[2]>, kMultiThreads, 0);
     if (e != hipSuccess) n = 1;
     return n > 0 ? n : 1;
 }

@@ -28,13 +28,21 @@ constexpr SymVariant kSymVariants[] = {
     {256, 1, 8192, 256, 8}, {1024, 1, 32768, 1024, 4},
 };
 constexpr int kSymMultiVariants = 9;
-constexpr int kSymSmallVariants = 4;  // variants 0..3 run in a multi-bin launch of their own
 constexpr int kSymMaxVariant = 9;
 constexpr int kSymMaxThreshold = 26214;  // distinct keys admitted in the 32768-slot table (80 %)
 constexpr int kSymMultiLds = 36 * 1024;  // bytes: max over variants 0..8 of r * sym_row_ints * 4
-// Waves per SIMD kept by the two multi-bin launches (small tables, larger
-// tables), as for kNumMinWaves.
-constexpr int kSymMultiMinWaves[2] = {7, 4};
+// The multi-bin variants run in three launches ("parts"), each with the
+// register and LDS budget of its own variants: small tables (0..3), mid
+// tables (4..6) and the two largest tables (7, 8; a row per workgroup).
+// A launch's LDS is its largest variant's, so keeping the 8192-slot table
+// apart lets the mid tables (the 96^3 stencil's rows) run at 5 waves per
+// SIMD instead of 4.  kSymMultiMinWaves is each part's residency floor,
+// as for kNumMinWaves.
+constexpr int kSymParts = 3;
+constexpr int kSymPartLo[kSymParts] = {0, 4, 7};
+constexpr int kSymPartHi[kSymParts] = {4, 7, 9};
+constexpr int kSymMultiMinWaves[kSymParts] = {7, 5, 4};
+constexpr int sym_part(int variant) { return variant < kSymPartHi[0] ? 0 : (variant < kSymPartHi[1] ? 1 : 2); }
 
 // Fused sorted-columns + numeric variants, one kernel each: all use 256
 // threads except 9 (a 1024-thread workgroup per row); rows above every
@@ -169,7 +177,7 @@ void launch_scan_apply(const ProbSet<V>& ps, const BinTable& num, DevMeta* meta,
                        RowItem* items, long long* gpool_off, long long* inst_nnz, int after_fail, hipStream_t s);
 
 // symbolic count: multi-bin launch over bins [0, n_multi) of `sym`
-// part 0: variants [0, kSymSmallVariants); part 1: the other multi-bin variants
+// part p: variants [kSymPartLo[p], kSymPartHi[p])
 template <typename V>
 void launch_sym_multi(int part, const ProbSet<V>& ps, const int2* items, int bin_cap, const BinTable& sym,
                       int n_multi, const DevMeta* meta, int grid, hipStream_t s);

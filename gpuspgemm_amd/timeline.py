@@ -31,7 +31,7 @@ def short_name(name: str) -> str:
     if m:
         return "symnum_multi" + m.group(1)
     if "k_sym_multi" in n:
-        return "sym_multi_small" if ", 0, 4>" in n else "sym_multi"
+        return "sym_multi_small" if ", 0, 4>" in n else ("sym_multi_large" if ", 7, 9>" in n else "sym_multi")
     m = re.match(r"(?:void )?(k_\w+)", n)
     return m.group(1)[2:] if m else n.split("(")[0][:32]
 
