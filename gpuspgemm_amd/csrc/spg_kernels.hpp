@@ -35,13 +35,14 @@ constexpr int kSymMultiLds = 36 * 1024;  // bytes: max over variants 0..8 of r *
 // register and LDS budget of its own variants: small tables (0..3), mid
 // tables (4..6) and the two largest tables (7, 8; a row per workgroup).
 // A launch's LDS is its largest variant's, so keeping the 8192-slot table
-// apart lets the mid tables (the 96^3 stencil's rows) run at 5 waves per
-// SIMD instead of 4.  kSymMultiMinWaves is each part's residency floor,
-// as for kNumMinWaves.
+// apart lets the mid tables (the 96^3 stencil's rows) run at 6 waves per
+// SIMD instead of 4 (80 VGPRs, 8 bytes spilled per lane; measured faster
+// than 5 waves without a spill).  kSymMultiMinWaves is each part's
+// residency floor, as for kNumMinWaves.
 constexpr int kSymParts = 3;
 constexpr int kSymPartLo[kSymParts] = {0, 4, 7};
 constexpr int kSymPartHi[kSymParts] = {4, 7, 9};
-constexpr int kSymMultiMinWaves[kSymParts] = {7, 5, 4};
+constexpr int kSymMultiMinWaves[kSymParts] = {7, 6, 4};
 constexpr int sym_part(int variant) { return variant < kSymPartHi[0] ? 0 : (variant < kSymPartHi[1] ? 1 : 2); }
 
 // Fused sorted-columns + numeric variants, one kernel each: all use 256
