@@ -90,7 +90,7 @@ def test_uniform_stencil_powerlaw(emu, tmp_path):
     run_emu(emu, tmp_path, A, A)
     S = gen.stencil27(g=5)
     run_emu(emu, tmp_path, S.astype(np.float32), S.astype(np.float32), "--f32")
-    P = gen.powerlaw(m=1500, seed=5)
+    P = gen.powerlaw(m=1000, seed=5)
     run_emu(emu, tmp_path, P, P, "--repeat", "2")
 
 
@@ -99,7 +99,7 @@ def test_graph_replay(emu, tmp_path):
     values (new addresses, same launches).  The first call runs eagerly, the
     second captures both phases, the rest replay them; every result must
     equal the host product."""
-    A = gen.powerlaw(m=1500, seed=5)
+    A = gen.powerlaw(m=1000, seed=5)
     out = run_emu(emu, tmp_path, A, A, "--repeat", "4", "--graphs")
     assert out.count("structure identical") == 4
     assert "graphs captured 2 replayed 4" in out
@@ -109,9 +109,9 @@ def test_graph_replay(emu, tmp_path):
     assert out.count("structure identical") == 3
     assert "graphs captured 2 replayed 2" in out
     # the server's lanes run eagerly with replay on
-    P = gen.powerlaw(m=800, seed=9)
-    out = run_emu(emu, tmp_path, P, P, "--serve", "9", str(1 << 30), "3", "--graphs")
-    assert out.count("structure identical") == 9
+    P = gen.powerlaw(m=600, seed=9)
+    out = run_emu(emu, tmp_path, P, P, "--serve", "6", str(1 << 30), "3", "--graphs")
+    assert out.count("structure identical") == 6
 
 
 def test_merged_launches(emu, tmp_path):
@@ -136,11 +136,13 @@ def _alternating_units_case():
     each in turn.  B has 16 entries per row; a row of A with k entries has
     NIP 16k."""
     nb = 400
-    brows = [np.unique(gen.rand_below(31, 1 + r, np.arange(16), np.full(16, 3000))) for r in range(nb)]
-    brows = [np.concatenate([x, np.arange(3000, 3000 + 16 - x.size)]) for x in brows]  # exactly 16 each
+    # columns drawn from 48: rows of C stay short (cheap fused pass), while
+    # the symbolic pass still hashes every product
+    brows = [np.unique(gen.rand_below(31, 1 + r, np.arange(16), np.full(16, 48))) for r in range(nb)]
+    brows = [np.concatenate([x, np.arange(48, 48 + 16 - x.size)]) for x in brows]  # exactly 16 each
     brpt = np.concatenate([[0], np.cumsum([x.size for x in brows])]).astype(np.int32)
     bcol = np.concatenate(brows).astype(np.int32)
-    B = HostCSR(nb, 3100, brpt, bcol, gen.rand_unit(32, 1, np.arange(bcol.size)) * 2 - 1)
+    B = HostCSR(nb, 64, brpt, bcol, gen.rand_unit(32, 1, np.arange(bcol.size)) * 2 - 1)
     lengths = [6] * 4 + [12] * 4 + [24] * 88  # NIP 96 (bin of v6), 192 (v5), 384 (v6)
     arows = [np.unique(gen.rand_below(33, 1 + i, np.arange(k), np.full(k, nb)))[:k] for i, k in enumerate(lengths)]
     arows = [np.concatenate([x, np.setdiff1d(np.arange(nb), x)[:k - x.size]]) for x, k in zip(arows, lengths)]
@@ -170,7 +172,7 @@ def test_symbolic_units_with_a_wave_held_back(emu, tmp_path):
     ft.write_text(table)
     res = subprocess.run([emu, str(fa), str(fb), "--sym-table", str(ft)], stdout=subprocess.PIPE,
                          stderr=subprocess.STDOUT, text=True, timeout=600,
-                         env=dict(os.environ, EMU_SLOW_WAVE="1:300"))
+                         env=dict(os.environ, EMU_SLOW_WAVE="1:100"))
     assert res.returncode == 0 and "structure identical" in res.stdout, res.stdout[-3000:]
 
 
@@ -263,15 +265,15 @@ def test_noalloc_entry(emu, tmp_path):
     reports the nnz needed, then repeated products match the host; the
     heavy-row case forced to the global kernels carves both global tables
     out of the fixed array too."""
-    P = gen.powerlaw(m=1500, seed=5)
+    P = gen.powerlaw(m=1000, seed=5)
     out = run_emu(emu, tmp_path, P, P, "--noalloc", "--repeat", "2")
     assert out.count("structure identical") == 2
     assert "status out of device memory" in out
     S = gen.stencil27(g=5).astype(np.float32)
     run_emu(emu, tmp_path, S, S, "--f32", "--noalloc")
     A, B = _heavy_row_case()
-    out = run_emu(emu, tmp_path, A, B, "--noalloc", "--repeat", "2", tables=GLOBAL_TABLES)
-    assert out.count("structure identical") == 2
+    out = run_emu(emu, tmp_path, A, B, "--noalloc", tables=GLOBAL_TABLES)
+    assert out.count("structure identical") == 1
 
 
 def test_server_batches_and_admission(emu, tmp_path):
@@ -280,12 +282,12 @@ def test_server_batches_and_admission(emu, tmp_path):
     so batches are split, admitted one after another and run on both
     lanes; every result must equal the host product.  A budget below one
     output must fail every request with out-of-memory."""
-    A = gen.powerlaw(m=3000, seed=8)
+    A = gen.powerlaw(m=1000, seed=8)
     C = oracle.spgemm(A, A)
     need = C.nnz * 12
-    out = run_emu(emu, tmp_path, A, A, "--serve", "7", str(2 * need + 64), "3")
-    assert out.count("structure identical") == 7
-    assert "served 7" in out
+    out = run_emu(emu, tmp_path, A, A, "--serve", "5", str(2 * need + 64), "2")
+    assert out.count("structure identical") == 5
+    assert "served 5" in out
     res = subprocess.run([emu, str(tmp_path / "a.csr"), str(tmp_path / "b.csr"), "--serve", "2", str(need // 2), "2"],
                          stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=600)
     assert res.returncode == 0, res.stdout[-3000:]
