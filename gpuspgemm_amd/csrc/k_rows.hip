@@ -14,9 +14,9 @@ __global__ __launch_bounds__(256) void k_clear(DevMeta* meta, int from_word) {
     for (int i = from_word + threadIdx.x; i < kWords; i += 256) w[i] = 0;
 }
 
-// NIP_i = sum over k in A(i,:) of nnz(B(k,:)) (setup.cuh:48-91): a thread
-// per row, or the whole wave on one row when a row of the wave has more
-// than 32 entries.  Rows with NIP 0 get nnz 0 right away and join no bin
+// NIP_i = sum over k in A(i,:) of nnz(B(k,:)) (setup.cuh:48-91): a lane
+// per row, each wave walking the entries of its 64 rows together (below).
+// Rows with NIP 0 get nnz 0 right away and join no bin
 // (binning.cuh:60-66).  The other rows are scattered into the symbolic bin
 // lists in the same pass (binning.cuh:76-120): bin b's list is the region
 // items[b * bin_cap, (b + 1) * bin_cap) with bin_cap >= the total row
@@ -30,6 +30,9 @@ __global__ __launch_bounds__(kRowThreads) void k_nip(ProbSet<V> ps, BinTable sym
     __shared__ int s_max;
     __shared__ unsigned long long s_total;
     __shared__ int s_flags;
+    constexpr int kU = 8;  // entries per lane and round
+    constexpr int kRound = kU * kWave;
+    __shared__ long long s_pref[kRowThreads / kWave][kRound];  // each wave's prefix sums of a round
     if (threadIdx.x < kMaxBins) s_cnt[threadIdx.x] = 0;
     if (threadIdx.x == 0) {
         s_max = 0;
@@ -40,59 +43,80 @@ __global__ __launch_bounds__(kRowThreads) void k_nip(ProbSet<V> ps, BinTable sym
 
     const Tile t = ps.tile(blockIdx.x);
     const GProb<V> P = ps.get(t.inst);
-    // The row of B (start, length) of every entry of the tile's rows is kept
-    // for the later passes.  In a tile of short rows (at most four entries
-    // on average, as in power-law matrices) each thread stores its rows'
-    // entries as it reads them below; in a tile of longer rows the threads
-    // first walk the tile's entries side by side (coalesced loads of A's
-    // columns and coalesced stores, four loads of each in flight): one row
-    // per thread would scatter the stores of a long row over many lines.
-    const int te0 = P.a_rpt[t.row0], te1 = P.a_rpt[t.row0 + t.rows];
-    const bool by_row = te1 - te0 <= 4 * t.rows;
-    if (!by_row) {
-        constexpr int kU = 4;
-        for (int base = te0 + threadIdx.x; base < te1; base += kU * kRowThreads) {
-            int kk[kU];
-#pragma unroll
-            for (int u = 0; u < kU; ++u) kk[u] = base + u * kRowThreads < te1 ? P.a_col[base + u * kRowThreads] : -1;
-#pragma unroll
-            for (int u = 0; u < kU; ++u)
-                if (kk[u] >= 0) {
-                    const int b0 = P.b_rpt[kk[u]], b1 = P.b_rpt[kk[u] + 1];
-                    P.bref[base + u * kRowThreads] = pack_bref(b0, b1 - b0);
-                }
-        }
-    }
+    // NIP of each lane's rows.  A wave takes 64 * kPer consecutive rows, kPer
+    // consecutive ones per lane, so the wave's entries are one consecutive
+    // range of A's columns.  The wave walks that range in rounds of kRound
+    // entries side by side: coalesced loads of A's columns, a gather of B's
+    // row pointers each (every load of the round in flight at once),
+    // coalesced stores of the (start, length) of every entry's row of B (the
+    // later passes read them).  The lengths then go through the wave's LDS
+    // slice, kU consecutive ones to a lane, into a prefix sum of the round,
+    // from which each lane takes its rows' shares.  Three dependent loads
+    // cover up to kRound entries however they fall over the rows.  Rows past
+    // the tile's end are empty rows at its last entry.
     constexpr int kPer = kTileRows / kRowThreads;
-    constexpr int kShort = 4;  // entries of a row read without a loop
-    // all loads of a thread's rows are issued together: row pointers, then
-    // up to kShort columns per row, then their B row lengths
-    int a0[kPer], a1[kPer];
+    long long* pref = s_pref[threadIdx.x / kWave];
+    int* lens = reinterpret_cast<int*>(pref);  // the round's lengths, in its first half
+    const int wrow = (threadIdx.x / kWave) * (kWave * kPer) + lane_id() * kPer;  // the lane's first row in the tile
+    int bound[kPer + 1];  // entry bounds of the lane's rows
 #pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-        const int r = threadIdx.x + j * kRowThreads;
-        a0[j] = a1[j] = 0;
-        if (r < t.rows) {
-            a0[j] = P.a_rpt[t.row0 + r];
-            a1[j] = P.a_rpt[t.row0 + r + 1];
+    for (int j = 0; j <= kPer; ++j) bound[j] = P.a_rpt[t.row0 + min(wrow + j, t.rows)];
+    long long s[kPer] = {};
+    const int g0 = __shfl(bound[0], 0), g1 = __shfl(bound[kPer], kWave - 1);
+    for (int rb = g0; rb < g1; rb += kRound) {
+        int kk[kU], b0[kU], len[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int e = rb + u * kWave + lane_id();
+            kk[u] = e < g1 ? P.a_col[e] : -1;
         }
-    }
-    int k[kPer][kShort];
+        // every gather is issued before the first store
 #pragma unroll
-    for (int j = 0; j < kPer; ++j)
-#pragma unroll
-        for (int q = 0; q < kShort; ++q) k[j][q] = a0[j] + q < a1[j] ? P.a_col[a0[j] + q] : -1;
-    long long s[kPer];
-#pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-        s[j] = 0;
-#pragma unroll
-        for (int q = 0; q < kShort; ++q)
-            if (k[j][q] >= 0) {
-                const int b0 = P.b_rpt[k[j][q]], b1 = P.b_rpt[k[j][q] + 1];
-                s[j] += b1 - b0;
-                if (by_row) P.bref[a0[j] + q] = pack_bref(b0, b1 - b0);
+        for (int u = 0; u < kU; ++u) {
+            b0[u] = 0;
+            len[u] = 0;
+            if (kk[u] >= 0) {
+                b0[u] = P.b_rpt[kk[u]];
+                len[u] = P.b_rpt[kk[u] + 1];
             }
+        }
+        wave_sync();  // the previous round's reads are done
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            len[u] -= b0[u];
+            if (kk[u] >= 0) P.bref[rb + u * kWave + lane_id()] = pack_bref(b0[u], len[u]);
+            lens[u * kWave + lane_id()] = len[u];
+        }
+        wave_sync();
+        long long loc[kU];
+        long long run = 0;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            run += lens[lane_id() * kU + u];
+            loc[u] = run;
+        }
+        // exclusive prefix of the lanes' totals
+        long long x = run;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const long long y = __shfl_up(x, o);
+            if (lane_id() >= o) x += y;
+        }
+        x -= run;
+        wave_sync();  // every lane has read its lengths
+#pragma unroll
+        for (int u = 0; u < kU; ++u) pref[lane_id() * kU + u] = x + loc[u];
+        wave_sync();
+        // pref[i] = sum of the round's lengths 0..i; a row's share is the
+        // difference at its clamped bounds
+        long long at[kPer + 1];
+#pragma unroll
+        for (int j = 0; j <= kPer; ++j) {
+            const int c = min(max(bound[j] - rb, 0), kRound);
+            at[j] = c > 0 ? pref[c - 1] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) s[j] += at[j + 1] - at[j];
     }
     int my_max = 0;
     long long my_total = 0;
@@ -100,47 +124,9 @@ __global__ __launch_bounds__(kRowThreads) void k_nip(ProbSet<V> ps, BinTable sym
     int bin[kPer], rank[kPer];
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
-        const int r = threadIdx.x + j * kRowThreads;
+        const int r = wrow + j;
         const bool valid = r < t.rows;
         const int row = t.row0 + r;
-        const int len = a1[j] - a0[j];
-        if (len > kShort && len <= 32)  // rest of the row, 8 entries' loads in flight at a time
-            for (int base = a0[j] + kShort; base < a1[j]; base += 8) {
-                int kk[8];
-#pragma unroll
-                for (int q = 0; q < 8; ++q) kk[q] = base + q < a1[j] ? P.a_col[base + q] : -1;
-#pragma unroll
-                for (int q = 0; q < 8; ++q)
-                    if (kk[q] >= 0) {
-                        const int b0 = P.b_rpt[kk[q]], b1 = P.b_rpt[kk[q] + 1];
-                        s[j] += b1 - b0;
-                        if (by_row) P.bref[base + q] = pack_bref(b0, b1 - b0);
-                    }
-            }
-        unsigned long long longs = __ballot(len > 32);
-        while (longs) {
-            const int src = __ffsll(static_cast<long long>(longs)) - 1;
-            longs &= longs - 1;
-            const int la0 = __shfl(a0[j], src);
-            const int la1 = __shfl(a1[j], src);
-            long long part = 0;
-            // 8 entries per lane in flight: a row of 2000 entries costs four
-            // rounds of two dependent loads instead of 32
-            for (int base = la0 + lane_id(); base < la1; base += 8 * kWave) {
-                int kk[8];
-#pragma unroll
-                for (int q = 0; q < 8; ++q) kk[q] = base + q * kWave < la1 ? P.a_col[base + q * kWave] : -1;
-#pragma unroll
-                for (int q = 0; q < 8; ++q)
-                    if (kk[q] >= 0) {
-                        const int b0 = P.b_rpt[kk[q]], b1 = P.b_rpt[kk[q] + 1];
-                        part += b1 - b0;
-                        if (by_row) P.bref[base + q * kWave] = pack_bref(b0, b1 - b0);
-                    }
-            }
-            part = wave_sum64(part);
-            if (lane_id() == src) s[j] = part;
-        }
         bin[j] = -1;
         if (valid) {
             if (s[j] > INT_MAX) {
@@ -178,7 +164,7 @@ __global__ __launch_bounds__(kRowThreads) void k_nip(ProbSet<V> ps, BinTable sym
     for (int j = 0; j < kPer; ++j)
         if (bin[j] >= 0)
             items[static_cast<long long>(bin[j]) * bin_cap + s_base[bin[j]] + rank[j]] =
-                make_int2(t.inst, t.row0 + threadIdx.x + j * kRowThreads);
+                make_int2(t.inst, t.row0 + wrow + j);
 }
 
 // Scan part 1: per-tile sum of nnz, NUM-bin histogram, max nnz and the

@@ -225,6 +225,14 @@ __device__ __forceinline__ void static_for(F&& f) {
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
+// Compiler-level ordering of LDS accesses among the lanes of one wave (a
+// wave's LDS instructions execute in order).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int T>
 __device__ __forceinline__ uint32_t hash_slot(int key) {
     static_assert((T & (T - 1)) == 0 && T >= 2, "table size must be a power of two");

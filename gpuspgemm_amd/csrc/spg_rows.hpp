@@ -22,14 +22,6 @@
 
 namespace spg {
 
-// Compiler-level ordering of LDS accesses among the lanes of one wave (a
-// wave's LDS instructions execute in order).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 template <int L>
 __device__ __forceinline__ void gsync() {
     if constexpr (L > kWave)

@@ -36,13 +36,24 @@ NUM_GLOBAL = next(v["id"] for v in _NV if v["global"])
 
 
 def build_emu(csrc: str, exe: str) -> None:
-    """The emulation build of the kernel and host sources in `csrc`."""
+    """The emulation build of the kernel and host sources in `csrc`: each
+    translation unit compiled by its own g++ process, side by side."""
     srcs = [os.path.join(csrc, f) for f in ("k_rows.hip", "k_sym.hip", "k_symnum.hip", "api.hip")]
-    cmd = ["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer", "-pthread",
-           "-Wno-unused-result", "-I", os.path.join(EMU, "include"), "-I", os.path.join(ROOT, "include"),
-           "-I", csrc, "-I", os.path.join(ROOT, "gpuspgemm_amd", "apps"), "-x", "c++", *srcs, "-x", "none",
-           os.path.join(EMU, "emu_runtime.cpp"), os.path.join(EMU, "emu_main.cpp"), "-o", exe]
-    subprocess.run(cmd, check=True)
+    srcs += [os.path.join(EMU, "emu_runtime.cpp"), os.path.join(EMU, "emu_main.cpp")]
+    flags = ["-std=c++17", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer", "-pthread",
+             "-Wno-unused-result"]
+    inc = ["-I", os.path.join(EMU, "include"), "-I", os.path.join(ROOT, "include"), "-I", csrc,
+           "-I", os.path.join(ROOT, "gpuspgemm_amd", "apps")]
+    objs = [f"{exe}.{i}.o" for i in range(len(srcs))]
+    procs = [subprocess.Popen(["g++", *flags, *inc, "-x", "c++", "-c", s, "-o", o]) for s, o in zip(srcs, objs)]
+    codes = [p.wait() for p in procs]
+    try:
+        assert codes == [0] * len(srcs), f"emulation build failed: {codes}"
+        subprocess.run(["g++", *flags, *objs, "-o", exe], check=True)
+    finally:
+        for o in objs:
+            if os.path.exists(o):
+                os.remove(o)
 
 
 @pytest.fixture(scope="module")
@@ -54,7 +65,7 @@ def emu():
     return EXE
 
 
-def run_emu(exe, tmp_path, A, B, *extra, tables=None):
+def run_emu(exe, tmp_path, A, B, *extra, tables=None, env=None):
     fa, fb = tmp_path / "a.csr", tmp_path / "b.csr"
     A.save_to_filename(str(fa))
     B.save_to_filename(str(fb))
@@ -63,7 +74,8 @@ def run_emu(exe, tmp_path, A, B, *extra, tables=None):
         f = tmp_path / f"table{phase}.txt"
         f.write_text(text)
         args += ["--sym-table" if phase == 0 else "--num-table", str(f)]
-    res = subprocess.run(args, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=600)
+    res = subprocess.run(args, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=600,
+                         env=None if env is None else {**os.environ, **env})
     assert res.returncode == 0, res.stdout[-4000:]
     assert "structure identical" in res.stdout
     return res.stdout
@@ -180,6 +192,40 @@ def test_rectangular_and_empty_rows(emu, tmp_path):
     A = gen.random_csr(120, 90, 0.05, seed=1, empty_rows=0.3)
     B = gen.random_csr(90, 70, 0.06, seed=2, empty_rows=0.2)
     run_emu(emu, tmp_path, A, B)
+
+
+def test_nip_per_row(emu, tmp_path):
+    """NIP per row, read through the symbolic bins: a table of one-wide NIP
+    ranges (bin b takes the rows of NIP b + 1) counts the rows of every NIP
+    up to 14, so a length credited to the wrong row changes a count.  Rows
+    of up to six entries sit beside rows of 700 and 1500 entries, whose
+    64-row groups take several rounds of the NIP pass, and the 3000 rows
+    end in a partial tile.  The debug log gives the counts, the total and
+    the maximum; the oracle's NIP per row gives the expected ones."""
+    rng = np.random.default_rng(41)
+    n = 4096
+    b_len = rng.integers(0, 4, n)
+    b_rpt = np.concatenate([[0], np.cumsum(b_len)]).astype(np.int32)
+    b_col = np.concatenate([np.sort(rng.choice(64, k, replace=False)) for k in b_len]).astype(np.int32)
+    B = HostCSR(n, 64, b_rpt, b_col, rng.uniform(-1, 1, b_col.size))
+    m = 3000
+    a_len = rng.integers(0, 7, m)
+    a_len[rng.random(m) < 0.2] = 0
+    a_len[[5, 1900, 2990]] = [700, 1500, 40]
+    a_rpt = np.concatenate([[0], np.cumsum(a_len)]).astype(np.int32)
+    a_col = np.concatenate([np.sort(rng.choice(n, k, replace=False)) for k in a_len]).astype(np.int32)
+    A = HostCSR(m, n, a_rpt, a_col, rng.uniform(-1, 1, a_col.size))
+    nip = oracle.nip_rows(A, B)
+    caps = list(range(1, 15))
+    table = "".join(f"0 {c} {INT_MAX}\n" for c in caps) + f"8 4000 {INT_MAX}\n{SYM_MAX} {INT_MAX} {INT_MAX}\n"
+    out = run_emu(emu, tmp_path, A, B, tables={0: table}, env={"SPG_LOG_LEVEL": "debug"})
+    want = [int(np.sum(nip == c)) for c in caps]
+    want += [int(np.sum((nip > 14) & (nip <= 4000))), int(np.sum(nip > 4000))]
+    assert want[-2] >= 2 and sum(want[:14]) > 1500, want
+    line = next(x for x in out.splitlines() if "symbolic phase" in x)
+    assert f"NIP {int(nip.sum())} (max {int(nip.max())} per row)" in line, line
+    got = line.split("rows per symbolic bin [")[1].split("]")[0]
+    assert [int(x) for x in got.split()] == want, line
 
 
 @pytest.mark.parametrize("variant", sorted(SYM))
