@@ -56,6 +56,12 @@ namespace {
 constexpr size_t kNoAllocScratch = size_t(256) << 20;
 static __device__ __align__(256) char scratch[kNoAllocScratch];
 
+// Products whose mean NIP per row, estimated from the operands' densities,
+// is at most this many queue the symbolic pass's small-table part first
+// (power-law 10^6 rows: 8.6, measured 0.325 -> 0.305 ms; the 96^3 stencil,
+// 704, keeps the other order: 3.9 ms against 4.4 with this one).
+constexpr double kSmallFirstNip = 32;
+
 thread_local std::string g_last_error;
 
 spg_status fail(spg_status s, const std::string& msg) {
@@ -788,6 +794,7 @@ struct Job : JobBase {
     std::vector<Tile> tiles;
     ProbSet<V> ps{};
     long long total_rows = 0;
+    bool small_first = false;  // the symbolic launches in the order of short products (front())
     int n_tiles = 0;
     bool batch = false;
     bool empty = false;
@@ -888,39 +895,54 @@ struct Job : JobBase {
         prof.begin("nip", total_rows, n_tiles, s);
         launch_nip<V>(ps, sym, meta, items1, static_cast<int>(total_rows), s);
         prof.end();
-        // The largest-table bin runs on an aux stream beside the multi-bin
-        // launches (queued first: its workgroups take the CUs they need
-        // before the multi-bin grids fill them, and those without a row
-        // exit at once); the part of the two largest multi-bin tables
-        // (rows of a workgroup each, few of them) follows it on that
-        // stream.  With the small-table part in use the mid-table part runs
-        // on the second aux stream: its workgroups take the CUs that the
-        // small-table grid leaves as its last units drain.  The launches
-        // count disjoint rows.
-        SPG_HIP(hipEventRecord(L.ev_ready, s));
-        SPG_HIP(hipStreamWaitEvent(L.aux[0], L.ev_ready, 0));
-        {
-            const int grid = c.cus * c.occ_sym_max;
-            prof.begin("sym_max", 0, grid, L.aux[0]);
-            launch_sym_max<V>(ps, items1, static_cast<int>(total_rows), sym, sym.n - 1, meta, fail_items, fail_off,
-                              grid, L.aux[0]);
-            prof.end();
-        }
+        // The launches count disjoint rows, on three streams: the part of the
+        // two largest multi-bin tables (rows of a workgroup each, few of
+        // them) on the first aux stream; with the small-table part in use,
+        // the mid-table part on the second aux stream (its workgroups take
+        // the CUs that the small-table grid leaves as its last units drain);
+        // the remaining part on the main stream.  The largest-table bin
+        // (139 KB of LDS a workgroup) goes where its workgroups find free
+        // CUs.  Rows of short products (small_first: a mean NIP per row of
+        // at most kSmallFirstNip, estimated from the operands' densities)
+        // fill the small-table part, the longest launch: it is queued first,
+        // right behind k_nip on the main stream, and the largest-table bin
+        // after it there.  Otherwise the largest-table bin is queued first,
+        // on the first aux stream, while the CUs are still free (its
+        // workgroups without a row exit at once; queued behind the
+        // multi-bin grids they were measured to hold back the mid-table
+        // launch of the stencil by a third), and its part follows it.
         const int n_multi = sym.n - 1;
         bool part_used[kSymParts] = {};
         for (int b = 0; b < n_multi; ++b) part_used[sym_part(sym.variant[b])] = true;
         const bool split = part_used[0] && part_used[1];
-        if (split) SPG_HIP(hipStreamWaitEvent(L.aux[1], L.ev_ready, 0));
         static const char* const kPartName[kSymParts] = {"sym_multi_small", "sym_multi", "sym_multi_large"};
-        for (int part : {2, 1, 0}) {
-            if (!part_used[part]) continue;
-            const hipStream_t where = part == 2 ? L.aux[0] : (split && part == 1 ? L.aux[1] : s);
+        auto part_stream = [&](int part) { return part == 2 ? L.aux[0] : (split && part == 1 ? L.aux[1] : s); };
+        auto queue_part = [&](int part) {
+            const hipStream_t where = part_stream(part);
             int grid = c.cus * c.occ_sym_multi[part];
             grid = (grid + kXcds - 1) / kXcds * kXcds;
             prof.begin(kPartName[part], total_rows, grid, where);
             launch_sym_multi<V>(part, ps, items1, static_cast<int>(total_rows), sym, n_multi, meta, grid, where);
             prof.end();
-        }
+        };
+        auto queue_max = [&](hipStream_t where) {
+            const int grid = c.cus * c.occ_sym_max;
+            prof.begin("sym_max", 0, grid, where);
+            launch_sym_max<V>(ps, items1, static_cast<int>(total_rows), sym, sym.n - 1, meta, fail_items, fail_off,
+                              grid, where);
+            prof.end();
+        };
+        SPG_HIP(hipEventRecord(L.ev_ready, s));
+        int main_part = -1;  // the part queued on the main stream
+        for (int part : {0, 1})
+            if (main_part < 0 && part_used[part] && part_stream(part) == s) main_part = part;
+        if (small_first && main_part >= 0) queue_part(main_part);
+        SPG_HIP(hipStreamWaitEvent(L.aux[0], L.ev_ready, 0));
+        if (!small_first) queue_max(L.aux[0]);
+        if (split) SPG_HIP(hipStreamWaitEvent(L.aux[1], L.ev_ready, 0));
+        for (int part : {2, 1, 0})
+            if (part_used[part] && !(small_first && part == main_part)) queue_part(part);
+        if (small_first) queue_max(s);
         SPG_HIP(join(L.aux[0], L.ev_done[0], s));
         if (split) SPG_HIP(join(L.aux[1], L.ev_done[1], s));
         return SPG_OK;
@@ -949,7 +971,7 @@ struct Job : JobBase {
     // contents (operand descriptors, tiles) are copied from pinned host
     // memory by the phase itself and so are not part of it.
     std::vector<int64_t> signature(int phase) const {
-        std::vector<int64_t> k{phase, static_cast<int64_t>(sizeof(V)), count, n_tiles, total_rows,
+        std::vector<int64_t> k{phase, static_cast<int64_t>(sizeof(V)), count, n_tiles, total_rows, small_first,
                                reinterpret_cast<int64_t>(L.scratch), reinterpret_cast<int64_t>(L.h_stage)};
         const BinTable& t = phase == 0 ? sym : num;
         k.push_back(t.n);
@@ -1026,6 +1048,12 @@ struct Job : JobBase {
             }
         }
         nnz.assign(count, 0);
+        {
+            double est_nip = 0;  // NIP of all rows if B's entries were spread evenly over its rows
+            for (int i = 0; i < count; ++i)
+                if (B[i].m > 0) est_nip += static_cast<double>(A[i].nnz) * B[i].nnz / B[i].m;
+            small_first = est_nip <= kSmallFirstNip * static_cast<double>(total_rows);
+        }
         // graph replay reads the descriptors from device memory in every call
         batch = count > 1 || graphs;
         if (total_rows >= INT_MAX) return failed(fail(SPG_ERR_OVERFLOW, "total row count exceeds int32"));
