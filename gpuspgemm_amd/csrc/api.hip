@@ -423,12 +423,17 @@ void lane_arm(Lane& L, int q) {
 }
 
 hipError_t lane_poll(Lane& L, int mask, const hipStream_t* streams) {
+    // The streams are queried every 100 us of waiting: a query can take
+    // ~10 us, and one in progress when the word arrives delays the return.
+    constexpr double kQueryEvery = 100e-6;
+    double next_query = now_s() + kQueryEvery;
     for (unsigned spin = 1;; ++spin) {
         bool all = true;
         for (int q = 0; q < Lane::kStreams; ++q)
             if ((mask >> q & 1) && __atomic_load_n(&L.h_done[q].flag, __ATOMIC_ACQUIRE) != 1) all = false;
         if (all) return hipSuccess;
-        if ((spin & 1023) == 0) {
+        if ((spin & 255) == 0 && now_s() >= next_query) {
+            next_query = now_s() + kQueryEvery;
             // a stream that has completed without storing its word failed;
             // the others keep being waited for
             for (int q = 0; q < Lane::kStreams; ++q) {
