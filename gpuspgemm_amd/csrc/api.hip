@@ -62,6 +62,12 @@ static __device__ __align__(256) char scratch[kNoAllocScratch];
 // 704, keeps the other order: 3.9 ms against 4.4 with this one).
 constexpr double kSmallFirstNip = 32;
 
+// Tile count (of 2048 rows) from which the scan stage scans the tile sums
+// once (k_tile_prefix, one workgroup: ~2 us per 1024 tiles) instead of
+// letting every workgroup sum those before it (tiles^2 / 2 reads: 2 M at
+// 2048 tiles, 2 * 10^9 at 2^27 rows).
+constexpr int kTilePrefixMin = 2048;
+
 thread_local std::string g_last_error;
 
 spg_status fail(spg_status s, const std::string& msg) {
@@ -461,6 +467,10 @@ struct Context {
     BinTable num{};
     int occ_sym_multi[kSymParts] = {1, 1, 1}, occ_sym_max = 1;
     int occ_num[2][kNumLdsVariants] = {};
+    // tiles from which the scan reads each tile's prefix off one scanned
+    // array instead of summing the earlier tiles per workgroup (quadratic
+    // in the tile count); SPG_TILE_PREFIX_MIN overrides it
+    int tile_prefix_min = kTilePrefixMin;
     Lane lane;  // graph replay and profiling run here, one call at a time
     Lane noalloc_lane;             // spg_multiply_noalloc, one call at a time
     char* noalloc_base = nullptr;  // this device's address of `scratch`
@@ -591,6 +601,7 @@ spg_status ensure_context(Context*& out) {
     SPG_TRY(check_num_table(c.num));
     for (int part = 0; part < kSymParts; ++part) c.occ_sym_multi[part] = occupancy_sym_multi<double>(part);
     c.occ_sym_max = occupancy_sym_max<double>();
+    if (const char* env = std::getenv("SPG_TILE_PREFIX_MIN")) c.tile_prefix_min = std::max(1, std::atoi(env));
     for (int v = 0; v < kNumLdsVariants; ++v) {
         c.occ_num[0][v] = occupancy_symnum_bin<float>(v);
         c.occ_num[1][v] = occupancy_symnum_bin<double>(v);
@@ -961,7 +972,8 @@ struct Job : JobBase {
         launch_scan_count<V>(ps, num, meta, tile_sums, after_fail, s);
         prof.end();
         prof.begin("scan_apply", total_rows, n_tiles, s);
-        launch_scan_apply<V>(ps, num, meta, tile_sums, items2, gpool_off, inst_nnz, after_fail, s);
+        launch_scan_apply<V>(ps, num, meta, tile_sums, items2, gpool_off, inst_nnz, after_fail, c.tile_prefix_min,
+                             s);
         prof.end();
         // counters (and nnz of every product of a batch) to pinned host memory, then the signal
         launch_readback(reinterpret_cast<const int*>(meta), reinterpret_cast<int*>(L.h_meta),

@@ -225,6 +225,37 @@ __global__ __launch_bounds__(kRowThreads) void k_scan_count(ProbSet<V> ps, BinTa
     }
 }
 
+// Between the two scan parts when a multiply has many tiles: the tile sums,
+// in place, become their exclusive prefix over all tiles (one workgroup,
+// 1024 tiles a round), so that k_scan_apply reads a tile's prefix instead
+// of summing every earlier tile of its product.
+__global__ __launch_bounds__(kRowThreads) void k_tile_prefix(long long* __restrict__ tile_sums, int n) {
+    constexpr int kWaves = kRowThreads / kWave;
+    __shared__ long long s_wave[kWaves];
+    long long carry = 0;
+    for (int base = 0; base < n; base += kRowThreads) {
+        const int i = base + threadIdx.x;
+        const long long v = i < n ? tile_sums[i] : 0;
+        long long incl = v;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const long long y = __shfl_up(incl, o);
+            if (lane_id() >= o) incl += y;
+        }
+        if (lane_id() == kWave - 1) s_wave[threadIdx.x / kWave] = incl;
+        __syncthreads();
+        long long before = 0, round = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            const long long x = s_wave[w];
+            if (w < static_cast<int>(threadIdx.x / kWave)) before += x;
+            round += x;
+        }
+        if (i < n) tile_sums[i] = carry + before + incl - v;
+        carry += round;
+        __syncthreads();  // s_wave is rewritten by the next round
+    }
+}
+
 // Scan part 2: exclusive prefix of nnz into c_rpt (binning.cuh:219-224),
 // c_rpt[m] = nnz(C), and the scatter of rows into the NUM bin lists.  The
 // prefix of a tile is the sum of the earlier tiles of its product, which
@@ -234,7 +265,8 @@ __global__ __launch_bounds__(kRowThreads) void k_scan_apply(ProbSet<V> ps, BinTa
                                                             const long long* __restrict__ tile_sums,
                                                             RowItem* __restrict__ items,
                                                             long long* __restrict__ gpool_off,
-                                                            long long* __restrict__ inst_nnz, int after_fail) {
+                                                            long long* __restrict__ inst_nnz, int after_fail,
+                                                            int tiles_scanned) {
     if (!after_fail && meta->fail_count.v > 0) return;
     constexpr int kPer = kTileRows / kRowThreads;  // consecutive rows per thread
     constexpr int kWaves = kRowThreads / kWave;
@@ -249,10 +281,16 @@ __global__ __launch_bounds__(kRowThreads) void k_scan_apply(ProbSet<V> ps, BinTa
     if (threadIdx.x < kMaxBins) s_cnt[threadIdx.x] = 0;
     if (threadIdx.x == 0) s_flags = 0;
 
-    // prefix over earlier tiles of this product
+    // prefix over earlier tiles of this product: read off the scanned tile
+    // sums (k_tile_prefix, many tiles), or summed here (few tiles; the
+    // threads of a workgroup read them side by side)
     long long pre = 0;
-    for (int i = t.first_tile + threadIdx.x; i < static_cast<int>(blockIdx.x); i += kRowThreads)
-        pre += tile_sums[i];
+    if (tiles_scanned) {
+        if (threadIdx.x == 0) pre = tile_sums[blockIdx.x] - tile_sums[t.first_tile];
+    } else {
+        for (int i = t.first_tile + threadIdx.x; i < static_cast<int>(blockIdx.x); i += kRowThreads)
+            pre += tile_sums[i];
+    }
     pre = wave_sum64(pre);
     if (lane_id() == 0) s_wave[threadIdx.x / kWave] = pre;
     __syncthreads();
@@ -432,18 +470,21 @@ void launch_scan_count(const ProbSet<V>& ps, const BinTable& num, DevMeta* meta,
 }
 
 template <typename V>
-void launch_scan_apply(const ProbSet<V>& ps, const BinTable& num, DevMeta* meta, const long long* tile_sums,
-                       RowItem* items, long long* gpool_off, long long* inst_nnz, int after_fail, hipStream_t s) {
+void launch_scan_apply(const ProbSet<V>& ps, const BinTable& num, DevMeta* meta, long long* tile_sums,
+                       RowItem* items, long long* gpool_off, long long* inst_nnz, int after_fail,
+                       int tile_prefix_min, hipStream_t s) {
+    const int scanned = ps.n_tiles >= tile_prefix_min ? 1 : 0;
+    if (scanned) hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(kRowThreads), 0, s, tile_sums, ps.n_tiles);
     hipLaunchKernelGGL(k_scan_apply<V>, dim3(ps.n_tiles), dim3(kRowThreads), 0, s, ps, num, meta, tile_sums,
-                       items, gpool_off, inst_nnz, after_fail);
+                       items, gpool_off, inst_nnz, after_fail, scanned);
 }
 
 #define SPG_INSTANTIATE_ROWS(V)                                                                        \
     template void launch_nip<V>(const ProbSet<V>&, const BinTable&, DevMeta*, int2*, int, hipStream_t); \
     template void launch_scan_count<V>(const ProbSet<V>&, const BinTable&, DevMeta*, long long*, int,  \
                                        hipStream_t);                                                   \
-    template void launch_scan_apply<V>(const ProbSet<V>&, const BinTable&, DevMeta*, const long long*, \
-                                       RowItem*, long long*, long long*, int, hipStream_t);
+    template void launch_scan_apply<V>(const ProbSet<V>&, const BinTable&, DevMeta*, long long*,       \
+                                       RowItem*, long long*, long long*, int, int, hipStream_t);
 
 SPG_INSTANTIATE_ROWS(float)
 SPG_INSTANTIATE_ROWS(double)

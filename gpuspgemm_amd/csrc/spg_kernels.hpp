@@ -173,9 +173,12 @@ void launch_nip(const ProbSet<V>& ps, const BinTable& sym, DevMeta* meta, int2* 
 template <typename V>
 void launch_scan_count(const ProbSet<V>& ps, const BinTable& num, DevMeta* meta, long long* tile_sums,
                        int after_fail, hipStream_t s);
+// with tile_prefix_min or more tiles, k_tile_prefix turns tile_sums into
+// their exclusive prefix first (in place)
 template <typename V>
-void launch_scan_apply(const ProbSet<V>& ps, const BinTable& num, DevMeta* meta, const long long* tile_sums,
-                       RowItem* items, long long* gpool_off, long long* inst_nnz, int after_fail, hipStream_t s);
+void launch_scan_apply(const ProbSet<V>& ps, const BinTable& num, DevMeta* meta, long long* tile_sums,
+                       RowItem* items, long long* gpool_off, long long* inst_nnz, int after_fail,
+                       int tile_prefix_min, hipStream_t s);
 
 // symbolic count: multi-bin launch over bins [0, n_multi) of `sym`
 // part p: variants [kSymPartLo[p], kSymPartHi[p])

@@ -228,6 +228,20 @@ def test_nip_per_row(emu, tmp_path):
     assert [int(x) for x in got.split()] == want, line
 
 
+def test_scanned_tile_prefix(emu, tmp_path):
+    """The scan stage reads each tile's prefix off one scanned array of tile
+    sums, the path of multiplies of 2048 tiles or more, forced here from one
+    tile: a product of three tiles and a server's batches of several
+    products (each product's prefix starts at its own first tile) must
+    equal the host product."""
+    env = {"SPG_TILE_PREFIX_MIN": "1"}
+    P = gen.powerlaw(m=4500, seed=11)
+    run_emu(emu, tmp_path, P, P, env=env)
+    Q = gen.powerlaw(m=600, seed=9)
+    out = run_emu(emu, tmp_path, Q, Q, "--serve", "4", str(1 << 30), "4", env=env)
+    assert out.count("structure identical") == 4
+
+
 @pytest.mark.parametrize("variant", sorted(SYM))
 def test_symbolic_variant(emu, tmp_path, variant):
     lanes, ec, t = SYM[variant]

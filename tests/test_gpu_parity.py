@@ -59,6 +59,16 @@ def test_powerlaw_full(gpu_lib, dtype):
     _say(st)
 
 
+def test_many_tiles_single_product(gpu_lib):
+    """5 * 10^6 rows (2442 tiles of 2048 rows): the scan stage reads each
+    tile's prefix off one scanned array of tile sums instead of summing the
+    earlier tiles per workgroup; structure and values against the oracle."""
+    A = gen.uniform8(m=5_000_000, per_row=2, seed=17, n=5_000_000)
+    _say("uniform m=5e6, 2 per row")
+    st = compare(A, A, gpu_multiply(A, A))
+    assert st["nnz"] > 0
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_stencil_small(gpu_lib, dtype):
     A = gen.stencil27(g=24).astype(dtype)
