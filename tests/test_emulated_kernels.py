@@ -7,6 +7,7 @@ host Gustavson product by tests/emu/emu_main.cpp.  Inputs are small:
 emulating a 1024-lane workgroup costs a thread per lane."""
 import os
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import pytest
@@ -81,6 +82,24 @@ def run_emu(exe, tmp_path, A, B, *extra, tables=None, env=None):
     return res.stdout
 
 
+def call(A, B, *extra, tables=None, env=None):
+    """One emulator run for run_emu_all."""
+    return A, B, extra, tables, env
+
+
+def run_emu_all(exe, tmp_path, *calls):
+    """Independent emulator runs side by side (each emulated workgroup keeps
+    only a few host cores busy), each in a directory of its own; their
+    outputs in order."""
+    def one(i):
+        A, B, extra, tables, env = calls[i]
+        d = tmp_path / f"run{i}"
+        d.mkdir()
+        return run_emu(exe, d, A, B, *extra, tables=tables, env=env)
+    with ThreadPoolExecutor(len(calls)) as ex:
+        return list(ex.map(one, range(len(calls))))
+
+
 def squares(m, k, seed):
     """k entries per row of A and of B: NIP = k^2, nnz <= k^2."""
     n = max(4 * k, 64)
@@ -99,11 +118,9 @@ def rows_for(lanes):
 
 def test_uniform_stencil_powerlaw(emu, tmp_path):
     A = gen.uniform8(m=512)
-    run_emu(emu, tmp_path, A, A)
-    S = gen.stencil27(g=5)
-    run_emu(emu, tmp_path, S.astype(np.float32), S.astype(np.float32), "--f32")
+    S = gen.stencil27(g=5).astype(np.float32)
     P = gen.powerlaw(m=1000, seed=5)
-    run_emu(emu, tmp_path, P, P, "--repeat", "2")
+    run_emu_all(emu, tmp_path, call(A, A), call(S, S, "--f32"), call(P, P, "--repeat", "2"))
 
 
 def test_graph_replay(emu, tmp_path):
@@ -112,18 +129,18 @@ def test_graph_replay(emu, tmp_path):
     second captures both phases, the rest replay them; every result must
     equal the host product."""
     A = gen.powerlaw(m=1000, seed=5)
-    out = run_emu(emu, tmp_path, A, A, "--repeat", "4", "--graphs")
-    assert out.count("structure identical") == 4
-    assert "graphs captured 2 replayed 4" in out
-    A = gen.random_csr(200, 150, 0.04, seed=6, empty_rows=0.3)
-    B = gen.random_csr(150, 120, 0.05, seed=7)
-    out = run_emu(emu, tmp_path, A, B, "--repeat", "3", "--graphs")
-    assert out.count("structure identical") == 3
-    assert "graphs captured 2 replayed 2" in out
-    # the server's lanes run eagerly with replay on
+    R = gen.random_csr(200, 150, 0.04, seed=6, empty_rows=0.3)
+    Rb = gen.random_csr(150, 120, 0.05, seed=7)
     P = gen.powerlaw(m=600, seed=9)
-    out = run_emu(emu, tmp_path, P, P, "--serve", "6", str(1 << 30), "3", "--graphs")
-    assert out.count("structure identical") == 6
+    out = run_emu_all(emu, tmp_path, call(A, A, "--repeat", "4", "--graphs"),
+                      call(R, Rb, "--repeat", "3", "--graphs"),
+                      # the server's lanes run eagerly with replay on
+                      call(P, P, "--serve", "6", str(1 << 30), "3", "--graphs"))
+    assert out[0].count("structure identical") == 4
+    assert "graphs captured 2 replayed 4" in out[0]
+    assert out[1].count("structure identical") == 3
+    assert "graphs captured 2 replayed 2" in out[1]
+    assert out[2].count("structure identical") == 6
 
 
 def test_merged_launches(emu, tmp_path):
@@ -136,8 +153,8 @@ def test_merged_launches(emu, tmp_path):
     table = ("10 4 4\n14 8 8\n11 12 16\n16 16 16\n12 32 32\n13 64 64\n"
              f"4 96 96\n5 160 128\n6 512 256\n7 {INT_MAX} 1024\n{NUM_GLOBAL} {INT_MAX} {INT_MAX}\n")
     A = gen.powerlaw(m=2500, seed=21)
-    run_emu(emu, tmp_path, A, A, tables={1: table})
-    run_emu(emu, tmp_path, A.astype(np.float32), A.astype(np.float32), "--f32", tables={1: table})
+    F = A.astype(np.float32)
+    run_emu_all(emu, tmp_path, call(A, A, tables={1: table}), call(F, F, "--f32", tables={1: table}))
 
 
 def _alternating_units_case():
@@ -236,10 +253,9 @@ def test_scanned_tile_prefix(emu, tmp_path):
     equal the host product."""
     env = {"SPG_TILE_PREFIX_MIN": "1"}
     P = gen.powerlaw(m=4500, seed=11)
-    run_emu(emu, tmp_path, P, P, env=env)
     Q = gen.powerlaw(m=600, seed=9)
-    out = run_emu(emu, tmp_path, Q, Q, "--serve", "4", str(1 << 30), "4", env=env)
-    assert out.count("structure identical") == 4
+    out = run_emu_all(emu, tmp_path, call(P, P, env=env), call(Q, Q, "--serve", "4", str(1 << 30), "4", env=env))
+    assert out[1].count("structure identical") == 4
 
 
 @pytest.mark.parametrize("variant", sorted(SYM))
@@ -249,8 +265,8 @@ def test_symbolic_variant(emu, tmp_path, variant):
     table = {0: f"{variant} {cap} {INT_MAX}\n{SYM_MAX} {INT_MAX} {INT_MAX}\n"}
     m = rows_for(lanes)
     k = int(np.sqrt(cap))
-    run_emu(emu, tmp_path, *squares(m, k, seed=variant), tables=table)
-    run_emu(emu, tmp_path, *chunked(m, min(cap, 2 * ec + 1), seed=variant), tables=table)
+    run_emu_all(emu, tmp_path, call(*squares(m, k, seed=variant), tables=table),
+                call(*chunked(m, min(cap, 2 * ec + 1), seed=variant), tables=table))
 
 
 @pytest.mark.parametrize("variant", sorted(NUM))
@@ -258,8 +274,8 @@ def test_fused_variant(emu, tmp_path, variant):
     lanes, ec, p = NUM[variant]
     table = {1: f"{variant} {INT_MAX} {p}\n{NUM_GLOBAL} {INT_MAX} {INT_MAX}\n"}
     m = rows_for(lanes)
-    run_emu(emu, tmp_path, *squares(m, int(np.sqrt(p)), seed=10 + variant), tables=table)
-    run_emu(emu, tmp_path, *chunked(m, min(p, 2 * ec + 1), seed=10 + variant), "--f32", tables=table)
+    run_emu_all(emu, tmp_path, call(*squares(m, int(np.sqrt(p)), seed=10 + variant), tables=table),
+                call(*chunked(m, min(p, 2 * ec + 1), seed=10 + variant), "--f32", tables=table))
 
 
 WAVE_ROWS = [(0, v) for v, (lanes, _, _) in sorted(SYM.items()) if lanes == 64] + \
@@ -286,8 +302,8 @@ def test_wave_rows_count_entry_starts(emu, tmp_path, phase, variant):
         ncol = min(p, 400)
         per_row = 2.5 * 64 * 12 / (120 * 0.6)
     B = gen.random_csr(400, ncol, min(0.9, per_row / ncol), seed=70 + variant, empty_rows=0.4)
-    run_emu(emu, tmp_path, A, B, tables=table)
-    run_emu(emu, tmp_path, A.astype(np.float32), B.astype(np.float32), "--f32", tables=table)
+    run_emu_all(emu, tmp_path, call(A, B, tables=table),
+                call(A.astype(np.float32), B.astype(np.float32), "--f32", tables=table))
 
 
 def _heavy_row_case():
@@ -315,8 +331,7 @@ def test_global_paths(emu, tmp_path):
     """The heavy-row case first with the default tables and then with every
     row forced to the global kernels."""
     A, B = _heavy_row_case()
-    run_emu(emu, tmp_path, A, B)
-    run_emu(emu, tmp_path, A, B, tables=GLOBAL_TABLES)
+    run_emu_all(emu, tmp_path, call(A, B), call(A, B, tables=GLOBAL_TABLES))
 
 
 def test_noalloc_entry(emu, tmp_path):
@@ -326,14 +341,13 @@ def test_noalloc_entry(emu, tmp_path):
     heavy-row case forced to the global kernels carves both global tables
     out of the fixed array too."""
     P = gen.powerlaw(m=1000, seed=5)
-    out = run_emu(emu, tmp_path, P, P, "--noalloc", "--repeat", "2")
-    assert out.count("structure identical") == 2
-    assert "status out of device memory" in out
     S = gen.stencil27(g=5).astype(np.float32)
-    run_emu(emu, tmp_path, S, S, "--f32", "--noalloc")
     A, B = _heavy_row_case()
-    out = run_emu(emu, tmp_path, A, B, "--noalloc", tables=GLOBAL_TABLES)
-    assert out.count("structure identical") == 1
+    out = run_emu_all(emu, tmp_path, call(P, P, "--noalloc", "--repeat", "2"), call(S, S, "--f32", "--noalloc"),
+                      call(A, B, "--noalloc", tables=GLOBAL_TABLES))
+    assert out[0].count("structure identical") == 2
+    assert "status out of device memory" in out[0]
+    assert out[2].count("structure identical") == 1
 
 
 def test_server_batches_and_admission(emu, tmp_path):
