@@ -429,8 +429,9 @@ void lane_arm(Lane& L, int q) {
 }
 
 hipError_t lane_poll(Lane& L, int mask, const hipStream_t* streams) {
-    // The streams are queried every 100 us of waiting: a query can take
-    // ~10 us, and one in progress when the word arrives delays the return.
+    // The streams are queried every 100 us of waiting: a query costs
+    // microseconds of host time (up to 11 us in an API trace), and one in
+    // progress when the word arrives delays the return.
     constexpr double kQueryEvery = 100e-6;
     double next_query = now_s() + kQueryEvery;
     for (unsigned spin = 1;; ++spin) {
