@@ -191,19 +191,22 @@ __global__ __launch_bounds__(kRowThreads) void k_scan_count(ProbSet<V> ps, BinTa
     long long sum = 0;
     unsigned long long pool = 0;
     int mx = 0;
+    constexpr int kPer = kTileRows / kRowThreads;
+    int nnz[kPer], nip[kPer];  // both loads of every row issued before any is used
 #pragma unroll
-    for (int j = 0; j < kTileRows / kRowThreads; ++j) {
+    for (int j = 0; j < kPer; ++j) {
         const int r = threadIdx.x + j * kRowThreads;
+        nnz[j] = r < t.rows ? P.c_rpt[t.row0 + r] : 0;
+        nip[j] = r < t.rows ? P.nip[t.row0 + r] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
         int b = -1;
-        if (r < t.rows) {
-            const int row = t.row0 + r;
-            const int nnz = P.c_rpt[row];
-            sum += nnz;
-            mx = max(mx, nnz);
-            if (nnz > 0) {
-                b = find_bin(num, P.nip[row], nnz);
-                if (num.variant[b] == kNumGlobalVariant) pool += global_table_slots(nnz);
-            }
+        sum += nnz[j];
+        mx = max(mx, nnz[j]);
+        if (nnz[j] > 0) {
+            b = find_bin(num, nip[j], nnz[j]);
+            if (num.variant[b] == kNumGlobalVariant) pool += global_table_slots(nnz[j]);
         }
         wave_count_add(s_cnt, b);
     }
@@ -281,6 +284,25 @@ __global__ __launch_bounds__(kRowThreads) void k_scan_apply(ProbSet<V> ps, BinTa
     if (threadIdx.x < kMaxBins) s_cnt[threadIdx.x] = 0;
     if (threadIdx.x == 0) s_flags = 0;
 
+    // Every load that does not depend on another is issued first, so that
+    // their latencies overlap: the rows' values, the bin sizes (for the
+    // offset of each bin's list) and, below, the earlier tiles' sums.
+    int v[kPer];
+    int nip[kPer];
+    int ar[kPer + 1];  // the rows' A row pointers, for the fused-pass items
+    const int r0 = threadIdx.x * kPer;  // consecutive rows per thread
+#pragma unroll
+    for (int j = 0; j <= kPer; ++j) ar[j] = r0 + j <= t.rows ? P.a_rpt[t.row0 + r0 + j] : 0;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int r = r0 + j;
+        v[j] = r < t.rows ? P.c_rpt[t.row0 + r] : 0;
+        nip[j] = r < t.rows ? P.nip[t.row0 + r] : 0;
+    }
+    int bin_off = 0;  // thread b < num.n: rows of the bins before b
+    if (threadIdx.x < num.n)
+        for (int b = 0; b < static_cast<int>(threadIdx.x); ++b) bin_off += meta->num_count[b].v;
+
     // prefix over earlier tiles of this product: read off the scanned tile
     // sums (k_tile_prefix, many tiles), or summed here (few tiles; the
     // threads of a workgroup read them side by side)
@@ -302,21 +324,13 @@ __global__ __launch_bounds__(kRowThreads) void k_scan_apply(ProbSet<V> ps, BinTa
     __syncthreads();
     const long long tile_base = s_prefix;
 
-    // thread-local values (consecutive rows)
-    int v[kPer];
+    // bins and ranks of the rows
     int bin[kPer];
     int rank[kPer];
-    int nip[kPer];
-    int ar[kPer + 1];  // the rows' A row pointers, for the fused-pass items
     long long local = 0;
-    const int r0 = threadIdx.x * kPer;
-#pragma unroll
-    for (int j = 0; j <= kPer; ++j) ar[j] = r0 + j <= t.rows ? P.a_rpt[t.row0 + r0 + j] : 0;
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
-        const int r = r0 + j;
-        v[j] = r < t.rows ? P.c_rpt[t.row0 + r] : 0;
-        nip[j] = v[j] > 0 ? P.nip[t.row0 + r] : 0;
+        if (v[j] == 0) nip[j] = 0;
         local += v[j];
         bin[j] = v[j] > 0 ? find_bin(num, nip[j], v[j]) : -1;
         rank[j] = wave_rank_add(s_cnt, bin[j]);
@@ -357,11 +371,8 @@ __global__ __launch_bounds__(kRowThreads) void k_scan_apply(ProbSet<V> ps, BinTa
     if (flags) atomicOr(&s_flags, flags);
     __syncthreads();
 
-    if (threadIdx.x < num.n && s_cnt[threadIdx.x] > 0) {
-        int off = 0;
-        for (int b = 0; b < static_cast<int>(threadIdx.x); ++b) off += meta->num_count[b].v;
-        s_base[threadIdx.x] = off + atomicAdd(&meta->num_cursor[threadIdx.x].v, s_cnt[threadIdx.x]);
-    }
+    if (threadIdx.x < num.n && s_cnt[threadIdx.x] > 0)
+        s_base[threadIdx.x] = bin_off + atomicAdd(&meta->num_cursor[threadIdx.x].v, s_cnt[threadIdx.x]);
     if (threadIdx.x == 0 && s_flags) atomicOr(&meta->flags.v, s_flags);
     __syncthreads();
 #pragma unroll
