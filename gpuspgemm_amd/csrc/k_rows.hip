@@ -387,27 +387,27 @@ __global__ __launch_bounds__(kRowThreads) void k_scan_apply(ProbSet<V> ps, BinTa
 }
 
 // Total NIP of A . B for the FLOP metric (nip.cuh:19-49): a tile of rows
-// per workgroup, a thread per row, one atomic per workgroup.
+// per workgroup, whose entries (one consecutive range of A's columns) the
+// threads walk side by side, eight loads each in flight; one atomic per
+// workgroup.
 __global__ __launch_bounds__(kRowThreads) void k_total_nip(const int32_t* __restrict__ a_rpt,
                                                            const int32_t* __restrict__ a_col,
                                                            const int32_t* __restrict__ b_rpt, int m,
                                                            unsigned long long* out) {
+    constexpr int kU = 8;
     __shared__ unsigned long long s_total;
     if (threadIdx.x == 0) s_total = 0;
     __syncthreads();
+    const int r0 = blockIdx.x * kTileRows;
+    const int e0 = a_rpt[r0], e1 = a_rpt[min(m, r0 + kTileRows)];
     long long s = 0;
-    for (int j = 0; j < kTileRows / kRowThreads; ++j) {
-        const int row = blockIdx.x * kTileRows + j * kRowThreads + threadIdx.x;
-        if (row >= m) continue;
-        const int a1 = a_rpt[row + 1];
-        for (int base = a_rpt[row]; base < a1; base += 8) {  // 8 entries' loads in flight
-            int kk[8];
+    for (int base = e0 + threadIdx.x; base < e1; base += kU * kRowThreads) {
+        int kk[kU];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) kk[q] = base + q < a1 ? a_col[base + q] : -1;
+        for (int u = 0; u < kU; ++u) kk[u] = base + u * kRowThreads < e1 ? a_col[base + u * kRowThreads] : -1;
 #pragma unroll
-            for (int q = 0; q < 8; ++q)
-                if (kk[q] >= 0) s += b_rpt[kk[q] + 1] - b_rpt[kk[q]];
-        }
+        for (int u = 0; u < kU; ++u)
+            if (kk[u] >= 0) s += b_rpt[kk[u] + 1] - b_rpt[kk[u]];
     }
     s = wave_sum64(s);
     if (lane_id() == 0 && s) atomicAdd(&s_total, static_cast<unsigned long long>(s));

@@ -260,6 +260,15 @@ def test_get_nip(gpu_lib):
     S = gen.stencil27(g=12)
     dS = S.to_device()
     assert g.get_nip(dS, dS) == int(oracle.nip_rows(S, S).sum())
+    # the bench's FLOP count: the full power-law product (489 tiles, the
+    # last one partial), and a rectangular product with empty rows
+    P = gen.powerlaw()
+    dP = P.to_device()
+    assert g.get_nip(dP, dP) == int(oracle.nip_rows(P, P).sum())
+    R = gen.random_csr(3000, 900, 0.01, seed=5, empty_rows=0.3)
+    Q = gen.random_csr(900, 700, 0.02, seed=6, empty_rows=0.2)
+    dR, dQ = R.to_device(), Q.to_device()
+    assert g.get_nip(dR, dQ) == int(oracle.nip_rows(R, Q).sum())
 
 
 def _batched_pair(i):
