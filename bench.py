@@ -13,7 +13,7 @@ Default workload: the power-law matrix (1,000,000 rows, Pareto(2) row
 lengths), FP64.  The stencil (96^3, 27-point) is measured beside it and
 reported under "secondary".
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 10] [--workload powerlaw]
+    python bench.py [--gpus 1] [--steps 30] [--warmup 20] [--workload powerlaw]
 """
 from __future__ import annotations
 
@@ -421,8 +421,8 @@ def copy_calibration(mbytes: int, reps: int = 5) -> None:
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--pause", type=float, default=0.0, help="ms of sleep before each call (reference: 50)")
     ap.add_argument("--workload", default="powerlaw", choices=["powerlaw", "stencil", "uniform8"])
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
